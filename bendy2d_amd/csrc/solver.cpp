@@ -1,0 +1,588 @@
+// solver.cpp -- the C ABI of include/bendy2d.h over the host scene and the
+// device engines.
+//
+// State model: HostScene is authoritative until the first bd_update after a
+// structural change (an add, a pin); the plan is then rebuilt and uploaded and
+// the device copy becomes authoritative.  Getters and structural changes first
+// copy device positions back (lazy readback).  Scalar fields (gravity, bounds,
+// sub_steps, link iterations) are passed with every update and need no upload.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/bendy2d.h"
+#include "engine.h"
+#include "scene.h"
+
+using namespace bd;
+
+namespace bd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+StepParams make_params(const HostScene &s, float dt, bool acc_pending) {
+    StepParams p;
+    float mult = 1.0f / (float)s.sub_steps;   // src/solver.rs:107
+    p.delta = dt * mult;                      // src/solver.rs:108
+    p.sub_steps = s.sub_steps;
+    p.link_iters = s.link_iters;
+    p.gx = s.gravity.x; p.gy = s.gravity.y;
+    p.gex = 0.0f + s.gravity.x;               // acc = (0,0) + g (src/particle.rs:52-54)
+    p.gey = 0.0f + s.gravity.y;
+    p.lox = s.bpos.x; p.loy = s.bpos.y;
+    p.hix = s.bpos.x + s.bsize.x;             // bounds.pos.x + bounds.size.x (src/particle.rs:221)
+    p.hiy = s.bpos.y + s.bsize.y;
+    p.first_acc = acc_pending;
+    return p;
+}
+
+}  // namespace bd
+
+struct bd_solver {
+    HostScene host;
+    Plan plan;
+    bool plan_valid = false;
+    bool dev_newer = false;    // device positions newer than host copy
+    bool acc_pending = false;  // initial circle / polygon-point acc not yet consumed
+    std::vector<uint8_t> center_from_prev;  // per polygon: centre = sum(prev)/n after an update
+    std::unique_ptr<Engine> eng;
+    int32_t engine_pref = BD_ENGINE_AUTO;
+    int32_t engine_used = 0;
+    hipStream_t stream = nullptr;
+    int device = -1;
+    Stats stats;
+};
+
+static int fail(int code, const std::string &msg) {
+    set_last_error(msg);
+    return code;
+}
+
+// Copy device state back into the host scene if the device is newer.
+static int ensure_host(const bd_solver *cs) {
+    bd_solver *s = const_cast<bd_solver *>(cs);
+    if (!s->dev_newer || !s->eng) return BD_OK;
+    std::string err;
+    int rc = s->eng->download(s->host, err);
+    if (rc) return fail(rc, err);
+    // Polygon::update recomputes the centre from the positions it is about to
+    // integrate (src/polygon.rs:130), which are prev afterwards.
+    for (size_t p = 0; p < s->host.polys.size(); p++) {
+        Poly &pg = s->host.polys[p];
+        if (pg.is_static || p >= s->center_from_prev.size() || !s->center_from_prev[p]) continue;
+        float cx = 0.0f, cy = 0.0f;
+        for (uint32_t i = 0; i < pg.n; i++) {
+            cx = cx + s->host.g_prev[pg.off + i].x;
+            cy = cy + s->host.g_prev[pg.off + i].y;
+        }
+        pg.center = V2{cx / (float)pg.n, cy / (float)pg.n};
+    }
+    s->dev_newer = false;
+    return BD_OK;
+}
+
+// Structural change: host must be current and the plan is rebuilt next update.
+static void begin_change(bd_solver *s) {
+    ensure_host(s);
+    s->plan_valid = false;
+}
+
+static uint32_t idx32(uint64_t v) { return v >= 0xffffffffull ? kBadIndex : (uint32_t)v; }
+
+extern "C" {
+
+const char *bd_error_string(int32_t code) {
+    switch (code) {
+    case BD_OK: return "ok";
+    case BD_ERR_ARG: return "invalid argument";
+    case BD_ERR_LINK: return "invalid link (particle_a must be < particle_b < count)";
+    case BD_ERR_INDEX: return "index out of range";
+    case BD_ERR_DEVICE: return "HIP device error";
+    case BD_ERR_NOMEM: return "out of memory";
+    case BD_ERR_CAPACITY: return "engine capacity exceeded";
+    case BD_ERR_COLLISION: return "circle pair schedule did not converge";
+    case BD_ERR_NODEVICE: return "no HIP device visible";
+    case BD_ERR_BUDGET: return "memory budget exceeded";
+    case BD_ERR_STATE: return "invalid state";
+    default: return "unknown error";
+    }
+}
+
+const char *bd_last_error(void) { return g_last_error.c_str(); }
+
+const char *bd_version(void) { return "bendy2d_amd 0.1 (gfx950)"; }
+
+int32_t bd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+bd_solver *bd_solver_new(void) {
+    try {
+        return new bd_solver();
+    } catch (...) {
+        set_last_error("allocation failed");
+        return nullptr;
+    }
+}
+
+void bd_solver_free(bd_solver *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->eng.reset();
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+bd_solver *bd_solver_clone(const bd_solver *s) {
+    if (!s) return nullptr;
+    if (ensure_host(s)) return nullptr;
+    bd_solver *c = bd_solver_new();
+    if (!c) return nullptr;
+    c->host = s->host;
+    c->acc_pending = false;
+    c->center_from_prev = s->center_from_prev;
+    c->engine_pref = s->engine_pref;
+    return c;
+}
+
+void bd_set_gravity(bd_solver *s, bd_vec2 g) { if (s) s->host.gravity = V2{g.x, g.y}; }
+bd_vec2 bd_get_gravity(const bd_solver *s) { return s ? bd_vec2{s->host.gravity.x, s->host.gravity.y} : bd_vec2{0, 0}; }
+void bd_set_bounds(bd_solver *s, bd_bounds b) {
+    if (!s) return;
+    s->host.bpos = V2{b.pos.x, b.pos.y};
+    s->host.bsize = V2{b.size.x, b.size.y};
+}
+bd_bounds bd_get_bounds(const bd_solver *s) {
+    bd_bounds b{{0, 0}, {0, 0}};
+    if (s) { b.pos = bd_vec2{s->host.bpos.x, s->host.bpos.y}; b.size = bd_vec2{s->host.bsize.x, s->host.bsize.y}; }
+    return b;
+}
+void bd_set_bounds_active(bd_solver *s, int32_t a) { if (s) s->host.bounds_active = a ? 1 : 0; }
+int32_t bd_get_bounds_active(const bd_solver *s) { return s ? s->host.bounds_active : 0; }
+
+int64_t bd_add_particle(bd_solver *s, bd_vec2 pos) {
+    if (!s) return BD_ERR_ARG;
+    begin_change(s);
+    s->host.p_pos.push_back(V2{pos.x, pos.y});
+    s->host.p_prev.push_back(V2{pos.x, pos.y});
+    s->host.p_pin.push_back(0);
+    return (int64_t)s->host.p_pos.size() - 1;
+}
+
+int64_t bd_ext_add_particle_with_prev(bd_solver *s, bd_vec2 pos, bd_vec2 prev) {
+    if (!s) return BD_ERR_ARG;
+    begin_change(s);
+    s->host.p_pos.push_back(V2{pos.x, pos.y});
+    s->host.p_prev.push_back(V2{prev.x, prev.y});
+    s->host.p_pin.push_back(0);
+    return (int64_t)s->host.p_pos.size() - 1;
+}
+
+int64_t bd_ext_add_particles(bd_solver *s, const bd_vec2 *pos, const bd_vec2 *prev, uint64_t n) {
+    if (!s || (!pos && n)) return BD_ERR_ARG;
+    begin_change(s);
+    int64_t first = (int64_t)s->host.p_pos.size();
+    s->host.p_pos.reserve(first + n); s->host.p_prev.reserve(first + n);
+    for (uint64_t i = 0; i < n; i++) {
+        s->host.p_pos.push_back(V2{pos[i].x, pos[i].y});
+        const bd_vec2 &q = prev ? prev[i] : pos[i];
+        s->host.p_prev.push_back(V2{q.x, q.y});
+    }
+    s->host.p_pin.resize(first + n, 0);
+    return first;
+}
+
+int64_t bd_add_circle(bd_solver *s, const bd_circle *c) {
+    if (!s || !c) return BD_ERR_ARG;
+    begin_change(s);
+    s->host.c_pos.push_back(V2{c->point.pos.x, c->point.pos.y});
+    s->host.c_prev.push_back(V2{c->point.prev_pos.x, c->point.prev_pos.y});
+    s->host.c_acc.push_back(V2{c->point.acc.x, c->point.acc.y});
+    s->host.c_r.push_back(c->radius);
+    return (int64_t)s->host.c_pos.size() - 1;
+}
+
+int64_t bd_ext_add_circles(bd_solver *s, const bd_circle *c, uint64_t n) {
+    if (!s || (!c && n)) return BD_ERR_ARG;
+    begin_change(s);
+    int64_t first = (int64_t)s->host.c_pos.size();
+    for (uint64_t i = 0; i < n; i++) {
+        s->host.c_pos.push_back(V2{c[i].point.pos.x, c[i].point.pos.y});
+        s->host.c_prev.push_back(V2{c[i].point.prev_pos.x, c[i].point.prev_pos.y});
+        s->host.c_acc.push_back(V2{c[i].point.acc.x, c[i].point.acc.y});
+        s->host.c_r.push_back(c[i].radius);
+    }
+    return first;
+}
+
+int64_t bd_add_particle_link(bd_solver *s, const bd_link *l) {
+    if (!s || !l) return BD_ERR_ARG;
+    begin_change(s);
+    s->host.plinks.push_back(LinkH{idx32(l->particle_a), idx32(l->particle_b), l->target_distance});
+    return (int64_t)s->host.plinks.size() - 1;
+}
+
+int64_t bd_add_circle_link(bd_solver *s, const bd_link *l) {
+    if (!s || !l) return BD_ERR_ARG;
+    begin_change(s);
+    s->host.clinks.push_back(LinkH{idx32(l->particle_a), idx32(l->particle_b), l->target_distance});
+    return (int64_t)s->host.clinks.size() - 1;
+}
+
+static int64_t add_links_bulk(bd_solver *s, std::vector<LinkH> &dst, const uint32_t *a, const uint32_t *b,
+                              const float *t, uint64_t n) {
+    if (!s || ((!a || !b || !t) && n)) return BD_ERR_ARG;
+    begin_change(s);
+    int64_t first = (int64_t)dst.size();
+    dst.reserve(first + n);
+    for (uint64_t i = 0; i < n; i++) dst.push_back(LinkH{a[i], b[i], t[i]});
+    return first;
+}
+
+int64_t bd_ext_add_particle_links(bd_solver *s, const uint32_t *a, const uint32_t *b, const float *t, uint64_t n) {
+    return s ? add_links_bulk(s, s->host.plinks, a, b, t, n) : BD_ERR_ARG;
+}
+
+int64_t bd_ext_add_circle_links(bd_solver *s, const uint32_t *a, const uint32_t *b, const float *t, uint64_t n) {
+    return s ? add_links_bulk(s, s->host.clinks, a, b, t, n) : BD_ERR_ARG;
+}
+
+int64_t bd_add_polygon(bd_solver *s, const bd_polygon_in *p) {
+    if (!s || !p || (!p->particles && p->n_particles) || (!p->particle_links && p->n_links)) return BD_ERR_ARG;
+    if (p->n_particles >= 0xffffffffull || p->n_links >= 0xffffffffull) return fail(BD_ERR_CAPACITY, "polygon too large");
+    begin_change(s);
+    HostScene &h = s->host;
+    Poly pg;
+    pg.off = (uint32_t)h.g_pos.size();
+    pg.n = (uint32_t)p->n_particles;
+    pg.loff = (uint32_t)h.glinks.size();
+    pg.nl = (uint32_t)p->n_links;
+    pg.is_static = p->is_static ? 1 : 0;
+    pg.center = V2{p->center.x, p->center.y};
+    pg.scale = p->scale;
+    for (uint64_t i = 0; i < p->n_particles; i++) {
+        const bd_particle &q = p->particles[i];
+        h.g_pos.push_back(V2{q.pos.x, q.pos.y});
+        h.g_prev.push_back(V2{q.prev_pos.x, q.prev_pos.y});
+        h.g_acc.push_back(V2{q.acc.x, q.acc.y});
+    }
+    for (uint64_t i = 0; i < p->n_links; i++) {
+        const bd_link &l = p->particle_links[i];
+        h.glinks.push_back(LinkH{idx32(l.particle_a), idx32(l.particle_b), l.target_distance});
+    }
+    h.polys.push_back(pg);
+    s->center_from_prev.push_back(0);
+    return (int64_t)h.polys.size() - 1;
+}
+
+int32_t bd_polygon_circle(float radius, bd_vec2 pos, uint64_t point_count, int32_t is_static,
+                          bd_particle *out_particles, bd_link *out_links, bd_vec2 *out_center) {
+    (void)is_static;
+    if ((!out_particles || !out_links) && point_count) return BD_ERR_ARG;
+    std::vector<V2> pts;
+    std::vector<LinkH> links;
+    V2 c;
+    polygon_circle(radius, V2{pos.x, pos.y}, point_count, pts, links, c);
+    for (uint64_t i = 0; i < pts.size(); i++)
+        out_particles[i] = bd_particle{{pts[i].x, pts[i].y}, {pts[i].x, pts[i].y}, {0.0f, 0.0f}};
+    for (uint64_t i = 0; i < links.size(); i++) out_links[i] = bd_link{links[i].a, links[i].b, links[i].t};
+    if (out_center) *out_center = bd_vec2{c.x, c.y};
+    return BD_OK;
+}
+
+int32_t bd_polygon_new(const bd_vec2 *points, uint64_t n, int32_t is_static, bd_particle *out_particles,
+                       bd_link *out_links, bd_vec2 *out_center) {
+    (void)is_static;
+    if ((!points || !out_particles || !out_links) && n) return BD_ERR_ARG;
+    std::vector<V2> in(n), pts;
+    for (uint64_t i = 0; i < n; i++) in[i] = V2{points[i].x, points[i].y};
+    std::vector<LinkH> links;
+    V2 c;
+    polygon_new(in.data(), n, pts, links, c);
+    for (uint64_t i = 0; i < pts.size(); i++)
+        out_particles[i] = bd_particle{{pts[i].x, pts[i].y}, {pts[i].x, pts[i].y}, {0.0f, 0.0f}};
+    for (uint64_t i = 0; i < links.size(); i++) out_links[i] = bd_link{links[i].a, links[i].b, links[i].t};
+    if (out_center) *out_center = bd_vec2{c.x, c.y};
+    return BD_OK;
+}
+
+uint64_t bd_get_particle_len(const bd_solver *s) { return s ? s->host.p_pos.size() : 0; }
+uint64_t bd_get_circles_len(const bd_solver *s) { return s ? s->host.c_pos.size() : 0; }
+uint64_t bd_get_polygons_len(const bd_solver *s) { return s ? s->host.polys.size() : 0; }
+
+static uint64_t copy_links(const std::vector<LinkH> &v, uint64_t begin, uint64_t count, bd_link *out, uint64_t cap) {
+    if (out)
+        for (uint64_t i = 0; i < count && i < cap; i++) {
+            const LinkH &l = v[begin + i];
+            out[i] = bd_link{l.a == kBadIndex ? ~0ull : l.a, l.b == kBadIndex ? ~0ull : l.b, l.t};
+        }
+    return count;
+}
+
+uint64_t bd_get_particle_links(const bd_solver *s, bd_link *out, uint64_t cap) {
+    return s ? copy_links(s->host.plinks, 0, s->host.plinks.size(), out, cap) : 0;
+}
+uint64_t bd_get_circle_links(const bd_solver *s, bd_link *out, uint64_t cap) {
+    return s ? copy_links(s->host.clinks, 0, s->host.clinks.size(), out, cap) : 0;
+}
+
+// acc of a body after updates: 0 except static polygon points, which keep
+// accumulating gravity (src/polygon.rs:126-128, 225-229).
+uint64_t bd_get_particles(const bd_solver *s, bd_particle *out, uint64_t cap) {
+    if (!s) return 0;
+    if (ensure_host(s)) return 0;
+    const HostScene &h = s->host;
+    if (out)
+        for (uint64_t i = 0; i < h.p_pos.size() && i < cap; i++)
+            out[i] = bd_particle{{h.p_pos[i].x, h.p_pos[i].y}, {h.p_prev[i].x, h.p_prev[i].y}, {0.0f, 0.0f}};
+    return h.p_pos.size();
+}
+
+uint64_t bd_get_circles(const bd_solver *s, bd_circle *out, uint64_t cap) {
+    if (!s) return 0;
+    if (ensure_host(s)) return 0;
+    const HostScene &h = s->host;
+    if (out)
+        for (uint64_t i = 0; i < h.c_pos.size() && i < cap; i++)
+            out[i] = bd_circle{{{h.c_pos[i].x, h.c_pos[i].y}, {h.c_prev[i].x, h.c_prev[i].y}, {h.c_acc[i].x, h.c_acc[i].y}},
+                               h.c_r[i]};
+    return h.c_pos.size();
+}
+
+int32_t bd_get_particle(const bd_solver *s, uint64_t i, bd_particle *out) {
+    if (!s || i >= s->host.p_pos.size()) return 0;
+    if (ensure_host(s)) return 0;
+    const HostScene &h = s->host;
+    if (out) *out = bd_particle{{h.p_pos[i].x, h.p_pos[i].y}, {h.p_prev[i].x, h.p_prev[i].y}, {0.0f, 0.0f}};
+    return 1;
+}
+
+int32_t bd_get_circle(const bd_solver *s, uint64_t i, bd_circle *out) {
+    if (!s || i >= s->host.c_pos.size()) return 0;
+    if (ensure_host(s)) return 0;
+    const HostScene &h = s->host;
+    if (out)
+        *out = bd_circle{{{h.c_pos[i].x, h.c_pos[i].y}, {h.c_prev[i].x, h.c_prev[i].y}, {h.c_acc[i].x, h.c_acc[i].y}},
+                         h.c_r[i]};
+    return 1;
+}
+
+int32_t bd_get_polygon(const bd_solver *s, uint64_t i, bd_polygon_info *info) {
+    if (!s || i >= s->host.polys.size()) return 0;
+    if (ensure_host(s)) return 0;
+    const Poly &p = s->host.polys[i];
+    if (info) *info = bd_polygon_info{p.n, p.nl, p.is_static, {p.center.x, p.center.y}, p.scale};
+    return 1;
+}
+
+uint64_t bd_get_polygon_particles(const bd_solver *s, uint64_t i, bd_particle *out, uint64_t cap) {
+    if (!s || i >= s->host.polys.size()) return 0;
+    if (ensure_host(s)) return 0;
+    const HostScene &h = s->host;
+    const Poly &p = h.polys[i];
+    if (out)
+        for (uint64_t k = 0; k < p.n && k < cap; k++) {
+            uint32_t g = p.off + (uint32_t)k;
+            out[k] = bd_particle{{h.g_pos[g].x, h.g_pos[g].y}, {h.g_prev[g].x, h.g_prev[g].y}, {h.g_acc[g].x, h.g_acc[g].y}};
+        }
+    return p.n;
+}
+
+uint64_t bd_get_polygon_links(const bd_solver *s, uint64_t i, bd_link *out, uint64_t cap) {
+    if (!s || i >= s->host.polys.size()) return 0;
+    const Poly &p = s->host.polys[i];
+    return copy_links(s->host.glinks, p.loff, p.nl, out, cap);
+}
+
+int32_t bd_ext_set_sub_steps(bd_solver *s, uint32_t n) {
+    if (!s || n < 1 || n > 65535) return BD_ERR_ARG;  // sub_steps is a u16 (src/solver.rs:29)
+    s->host.sub_steps = n;
+    return BD_OK;
+}
+uint32_t bd_ext_get_sub_steps(const bd_solver *s) { return s ? s->host.sub_steps : 0; }
+
+int32_t bd_ext_set_link_iterations(bd_solver *s, uint32_t k) {
+    if (!s || k < 1) return BD_ERR_ARG;
+    s->host.link_iters = k;
+    return BD_OK;
+}
+uint32_t bd_ext_get_link_iterations(const bd_solver *s) { return s ? s->host.link_iters : 0; }
+
+int32_t bd_ext_set_pinned(bd_solver *s, uint64_t i, int32_t pinned) {
+    if (!s) return BD_ERR_ARG;
+    if (i >= s->host.p_pos.size()) return BD_ERR_INDEX;
+    begin_change(s);
+    s->host.p_pin[i] = pinned ? 1 : 0;
+    return BD_OK;
+}
+
+int32_t bd_ext_set_pinned_range(bd_solver *s, uint64_t first, const uint8_t *pinned, uint64_t n) {
+    if (!s || (!pinned && n)) return BD_ERR_ARG;
+    if (first + n > s->host.p_pos.size()) return BD_ERR_INDEX;
+    begin_change(s);
+    for (uint64_t i = 0; i < n; i++) s->host.p_pin[first + i] = pinned[i] ? 1 : 0;
+    return BD_OK;
+}
+
+int64_t bd_ext_add_static_circle(bd_solver *s, bd_vec2 c, float r) {
+    if (!s) return BD_ERR_ARG;
+    begin_change(s);
+    s->host.cols.push_back(Collider{0, c.x, c.y, r, 0, 0});
+    return (int64_t)s->host.cols.size() - 1;
+}
+
+int64_t bd_ext_add_static_polygon(bd_solver *s, const bd_vec2 *v, uint64_t n) {
+    if (!s || !v || n < 3) return BD_ERR_ARG;
+    begin_change(s);
+    std::vector<V2> pts(n);
+    for (uint64_t i = 0; i < n; i++) pts[i] = V2{v[i].x, v[i].y};
+    uint32_t off = (uint32_t)(s->host.planes.size() / 3);
+    polygon_planes(pts.data(), n, s->host.planes);
+    for (uint64_t i = 0; i < n; i++) { s->host.verts.push_back(v[i].x); s->host.verts.push_back(v[i].y); }
+    s->host.cols.push_back(Collider{1, 0.0f, 0.0f, 0.0f, off, (uint32_t)n});
+    return (int64_t)s->host.cols.size() - 1;
+}
+
+uint64_t bd_ext_get_static_collider_len(const bd_solver *s) { return s ? s->host.cols.size() : 0; }
+
+int32_t bd_ext_get_static_collider(const bd_solver *s, uint64_t i, bd_vec2 *center, float *radius,
+                                   bd_vec2 *vertices, uint64_t cap, uint64_t *n_vertices) {
+    if (!s) return BD_ERR_ARG;
+    if (i >= s->host.cols.size()) return BD_ERR_INDEX;
+    const Collider &c = s->host.cols[i];
+    if (center) *center = bd_vec2{c.cx, c.cy};
+    if (radius) *radius = c.r;
+    if (n_vertices) *n_vertices = c.kind == 1 ? c.nv : 0;
+    if (c.kind == 1 && vertices)
+        for (uint32_t k = 0; k < c.nv && k < cap; k++)
+            vertices[k] = bd_vec2{s->host.verts[2 * (c.plane_off + k)], s->host.verts[2 * (c.plane_off + k) + 1]};
+    return c.kind;
+}
+
+int32_t bd_ext_get_pinned_range(const bd_solver *s, uint64_t first, uint8_t *out, uint64_t n) {
+    if (!s || (!out && n)) return BD_ERR_ARG;
+    if (first + n > s->host.p_pin.size()) return BD_ERR_INDEX;
+    for (uint64_t k = 0; k < n; k++) out[k] = s->host.p_pin[first + k];
+    return BD_OK;
+}
+
+int32_t bd_ext_set_engine(bd_solver *s, int32_t engine) {
+    if (!s || engine < BD_ENGINE_AUTO || engine > BD_ENGINE_TILED) return BD_ERR_ARG;
+    begin_change(s);
+    s->engine_pref = engine;
+    return BD_OK;
+}
+
+int32_t bd_ext_get_engine_used(const bd_solver *s) { return s ? s->engine_used : 0; }
+
+int64_t bd_plan_layers(const uint32_t *a, const uint32_t *b, uint64_t n_links, uint64_t n_points, uint32_t *layer,
+                       uint64_t *order) {
+    if ((!a || !b || !layer) && n_links) return BD_ERR_ARG;
+    for (uint64_t i = 0; i < n_links; i++)
+        if (a[i] >= n_points || b[i] >= n_points) return BD_ERR_INDEX;
+    return (int64_t)plan_layers(a, b, n_links, n_points, layer, order);
+}
+
+static int ensure_device(bd_solver *s) {
+    if (s->stream) return BD_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(BD_ERR_NODEVICE, "no HIP device visible");
+    if (hipGetDevice(&s->device) != hipSuccess) return fail(BD_ERR_DEVICE, "hipGetDevice failed");
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(BD_ERR_DEVICE, "hipStreamCreate failed");
+    return BD_OK;
+}
+
+int32_t bd_update(bd_solver *s, float dt) {
+    if (!s) return BD_ERR_ARG;
+    std::string err;
+    if (!s->plan_valid) {
+        int rc = ensure_host(s);
+        if (rc) return rc;
+        rc = build_plan(s->host, s->plan, err);
+        if (rc) return fail(rc, err);
+        rc = ensure_device(s);
+        if (rc) return rc;
+        int want = s->engine_pref;
+        if (want == BD_ENGINE_AUTO) {
+            if (scene_engine_fits(s->host, s->plan)) want = BD_ENGINE_SCENE;
+            else if (tiled_engine_fits(s->host, s->plan)) want = BD_ENGINE_TILED;
+            else want = BD_ENGINE_GLOBAL;
+        }
+        if (want == BD_ENGINE_SCENE && !scene_engine_fits(s->host, s->plan))
+            return fail(BD_ERR_CAPACITY, "scene does not fit the SCENE engine");
+        if (want == BD_ENGINE_TILED && !tiled_engine_fits(s->host, s->plan))
+            return fail(BD_ERR_CAPACITY, "link graph does not suit the TILED engine");
+        if (!s->eng || s->eng->kind() != want) {
+            s->eng.reset();
+            if (want == BD_ENGINE_SCENE) s->eng.reset(make_scene_engine(s->stream));
+            else if (want == BD_ENGINE_TILED) s->eng.reset(make_tiled_engine(s->stream));
+            else s->eng.reset(make_global_engine(s->stream));
+        }
+        rc = s->eng->upload(s->host, s->plan, err);
+        if (rc) return fail(rc, err);
+        s->plan_valid = true;
+        s->acc_pending = s->host.any_initial_acc();
+        s->engine_used = want;
+    }
+    StepParams sp = make_params(s->host, dt, s->acc_pending);
+    int rc = s->eng->update(sp, s->stats, err);
+    if (rc) return fail(rc, err);
+    s->stats.updates++;
+    // acc after update_positions: zero, except static polygons, which skip
+    // Polygon::update and keep adding gravity (src/polygon.rs:125-128).
+    HostScene &h = s->host;
+    for (V2 &a : h.c_acc) a = V2{0.0f, 0.0f};
+    for (size_t p = 0; p < h.polys.size(); p++) {
+        const Poly &pg = h.polys[p];
+        for (uint32_t i = 0; i < pg.n; i++) {
+            V2 &a = h.g_acc[pg.off + i];
+            if (pg.is_static) {
+                for (uint32_t k = 0; k < h.sub_steps; k++) a = V2{a.x + h.gravity.x, a.y + h.gravity.y};
+            } else {
+                a = V2{0.0f, 0.0f};
+            }
+        }
+        if (p < s->center_from_prev.size()) s->center_from_prev[p] = pg.is_static ? 0 : 1;
+    }
+    s->acc_pending = false;
+    s->dev_newer = true;
+    return BD_OK;
+}
+
+int32_t bd_ext_sync(bd_solver *s) {
+    if (!s) return BD_ERR_ARG;
+    if (!s->eng) return BD_OK;
+    std::string err;
+    int rc = s->eng->sync(err);
+    return rc ? fail(rc, err) : BD_OK;
+}
+
+int32_t bd_ext_get_stats(const bd_solver *s, bd_stats *out) {
+    if (!s || !out) return BD_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->n_points = s->host.n_points();
+    out->n_links = s->host.n_links();
+    out->link_layers = s->plan_valid ? s->plan.depth : 0;
+    out->updates = s->stats.updates;
+    out->substeps = s->stats.substeps;
+    out->kernel_launches = s->stats.launches;
+    out->cc_rounds = s->stats.cc_rounds;
+    out->cc_redos = s->stats.cc_redos;
+    out->cc_pairs = s->stats.cc_pairs;
+    out->cc_margin = s->stats.cc_margin;
+    out->engine = s->engine_used;
+    out->tile_passes = s->stats.tile_passes;
+    return BD_OK;
+}
+
+}  // extern "C"
