@@ -66,6 +66,12 @@ class GridDesc(Structure):
                 ("as_circles", c_int32), ("radius", c_float)]
 
 
+class BatchConfig(Structure):
+    _fields_ = [("wg_small", c_uint32), ("wg_large", c_uint32), ("split_points", c_uint32),
+                ("lds_small", c_uint32), ("lds_large", c_uint32), ("steps_per_launch", c_uint32),
+                ("use_graph", c_int32)]
+
+
 BD_ORDER_COLOUR = 0
 BD_ORDER_PARTICLE = 1
 
@@ -139,6 +145,19 @@ _SIGS = [
     ("bd_scene_add_gas", c_int64, [_S, c_uint64, c_float, c_float, c_float, c_float, c_float, c_uint64]),
     ("bd_scene_gas_box", c_int32, [_S, c_uint64, c_float, c_float, c_uint32, c_uint64]),
     ("bd_scene_softbody", c_int64, [_S, c_uint64, c_uint32]),
+    ("bd_batch_new", _S, []),
+    ("bd_batch_free", None, [_S]),
+    ("bd_batch_add_scene", c_int64, [_S, _S]),
+    ("bd_batch_len", c_uint64, [_S]),
+    ("bd_batch_total_points", c_uint64, [_S]),
+    ("bd_batch_update", c_int32, [_S, c_float, c_uint32]),
+    ("bd_batch_sync", c_int32, [_S]),
+    ("bd_batch_read_scene", c_int32, [_S, c_int64, _S]),
+    ("bd_batch_set_config", c_int32, [_S, POINTER(BatchConfig)]),
+    ("bd_batch_get_config", c_int32, [_S, POINTER(BatchConfig)]),
+    ("bd_batch_autotune", c_int32, [_S, c_float, c_uint32]),
+    ("bd_batch_last_timing", c_int32, [_S, POINTER(c_float), POINTER(c_uint32)]),
+    ("bd_batch_cc_stats", c_int32, [_S, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     ("bd_plan_layers", c_int64, [c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p]),
 ]
 

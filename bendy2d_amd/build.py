@@ -21,9 +21,11 @@ SOURCES = [
     "scene.cpp",
     "scenes.cpp",
     "solver.cpp",
+    "batch.cpp",
     "util.hip",
     "engine_global.hip",
     "engine_scene.hip",
+    "engine_tiled.hip",
 ]
 
 COMMON = [

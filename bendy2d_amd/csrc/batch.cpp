@@ -1,0 +1,510 @@
+// batch.cpp -- the batched form: many independent scenes resident in device
+// memory, advanced by persistent SCENE-engine launches (one per size class),
+// replayed as a captured hipGraph, with an autotuner over the launch shape.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/bendy2d.h"
+#include "batch.h"
+#include "engine.h"
+#include "kernels.h"
+#include "scene_pack.h"
+
+using namespace bd;
+
+namespace bd {
+
+static int fail(int code, const std::string &msg) {
+    set_last_error(msg);
+    return code;
+}
+
+BatchCore::~BatchCore() { release(); }
+
+void BatchCore::release() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
+    ev0 = ev1 = ev_fork = ev_join = nullptr;
+    side = nullptr;
+    stream = nullptr;
+}
+
+int BatchCore::init_device(std::string &err) {
+    if (stream) return BD_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) { err = "no HIP device visible"; return BD_ERR_NODEVICE; }
+    BD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    own_stream = true;
+    BD_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    BD_HIP(hipEventCreate(&ev0));
+    BD_HIP(hipEventCreate(&ev1));
+    BD_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    BD_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    hipDeviceProp_t prop;
+    BD_HIP(hipGetDeviceProperties(&prop, 0));
+    num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    return scene_kernels_init(err);
+}
+
+void BatchCore::invalidate_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+}
+
+// Split active scenes into the two size classes, each ordered largest first
+// (the work queue hands the longest scenes out first).
+void BatchCore::build_order(const std::vector<uint32_t> &active) {
+    std::vector<uint32_t> small, large;
+    for (uint32_t i : active) (descs[i].N <= cfg.split_points ? small : large).push_back(i);
+    auto by_cost = [&](uint32_t a, uint32_t b) {
+        uint64_t ca = (uint64_t)descs[a].N + descs[a].L, cb = (uint64_t)descs[b].N + descs[b].L;
+        return ca != cb ? ca > cb : a < b;
+    };
+    std::sort(small.begin(), small.end(), by_cost);
+    std::sort(large.begin(), large.end(), by_cost);
+    order.clear();
+    order.insert(order.end(), small.begin(), small.end());
+    order.insert(order.end(), large.begin(), large.end());
+    n_small = (uint32_t)small.size();
+    n_large = (uint32_t)large.size();
+    order_dirty = true;
+    invalidate_graph();
+}
+
+int BatchCore::upload_tables(std::string &err) {
+    size_t nd = std::max<size_t>(descs.size(), 1);
+    if (d_desc_cap < nd) {
+        d_desc.release(); d_order.release();
+        BD_HIP(d_desc.alloc(nd * sizeof(SceneDesc)));
+        BD_HIP(d_order.alloc(nd * sizeof(uint32_t)));
+        d_desc_cap = nd;
+        invalidate_graph();
+    }
+    if (!d_queue.p) BD_HIP(d_queue.alloc(64));
+    if (descs_dirty && !descs.empty()) {
+        BD_HIP(hipMemcpyAsync(d_desc.p, descs.data(), descs.size() * sizeof(SceneDesc), hipMemcpyHostToDevice, stream));
+        descs_dirty = false;
+    }
+    if (order_dirty && !order.empty()) {
+        BD_HIP(hipMemcpyAsync(d_order.p, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        order_dirty = false;
+    }
+    // pageable sources: make sure the copies are done before vectors change
+    BD_HIP(hipStreamSynchronize(stream));
+    return BD_OK;
+}
+
+// One launch of every size class: reset the queues (a kernel, never a
+// memset node), then the two persistent kernels on two streams.
+int BatchCore::enqueue_launch(uint32_t steps, std::string &err) {
+    launch_fill_u32((uint32_t *)d_queue.p, 2, 0u, stream);
+    int vs = variant_for_wg((int)cfg.wg_small), vl = variant_for_wg((int)cfg.wg_large);
+    if (vs < 0 || vl < 0) { err = "no kernel variant for the configured workgroup size"; return BD_ERR_ARG; }
+    const SceneDesc *dd = (const SceneDesc *)d_desc.p;
+    const uint32_t *od = (const uint32_t *)d_order.p;
+    uint32_t *q = (uint32_t *)d_queue.p;
+    BD_HIP(hipEventRecord(ev_fork, stream));
+    BD_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    if (n_large) {
+        uint32_t blocks = std::min<uint32_t>(n_large, (uint32_t)(num_cu * scene_blocks_per_cu(vl, cfg.lds_large)));
+        int rc = launch_scene_variant(vl, std::max<uint32_t>(blocks, 1), cfg.lds_large, side, dd, states_dev(), od + n_small,
+                                      n_large, q + 1, steps, err);
+        if (rc) return rc;
+    }
+    if (n_small) {
+        uint32_t blocks = std::min<uint32_t>(n_small, (uint32_t)(num_cu * scene_blocks_per_cu(vs, cfg.lds_small)));
+        int rc = launch_scene_variant(vs, std::max<uint32_t>(blocks, 1), cfg.lds_small, stream, dd, states_dev(), od,
+                                      n_small, q, steps, err);
+        if (rc) return rc;
+    }
+    BD_HIP(hipEventRecord(ev_join, side));
+    BD_HIP(hipStreamWaitEvent(stream, ev_join, 0));
+    return BD_OK;
+}
+
+// Run `launches` launches of `steps` updates each; replays a captured graph
+// when configured.
+int BatchCore::run(uint32_t launches, uint32_t steps, std::string &err) {
+    if (!launches) return BD_OK;
+    int rc = upload_tables(err);
+    if (rc) return rc;
+    BD_HIP(hipEventRecord(ev0, stream));
+    if (cfg.use_graph) {
+        if (!graph_exec || graph_steps != steps) {
+            invalidate_graph();
+            hipGraph_t g;
+            BD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_launch(steps, err);
+            hipGraph_t gg = nullptr;
+            hipError_t e = hipStreamEndCapture(stream, &gg);
+            if (rc) { if (gg) (void)hipGraphDestroy(gg); return rc; }
+            if (e != hipSuccess) { err = std::string("graph capture: ") + hipGetErrorString(e); return BD_ERR_DEVICE; }
+            g = gg;
+            e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { graph_exec = nullptr; err = std::string("graph instantiate: ") + hipGetErrorString(e); return BD_ERR_DEVICE; }
+            graph_steps = steps;
+        }
+        for (uint32_t l = 0; l < launches; l++) BD_HIP(hipGraphLaunch(graph_exec, stream));
+    } else {
+        for (uint32_t l = 0; l < launches; l++) {
+            rc = enqueue_launch(steps, err);
+            if (rc) return rc;
+        }
+    }
+    BD_HIP(hipEventRecord(ev1, stream));
+    timing_pending = true;
+    last_launches = launches * (1 + (n_small ? 1 : 0) + (n_large ? 1 : 0));
+    return BD_OK;
+}
+
+int BatchCore::finish_timing(std::string &err) {
+    if (!timing_pending) return BD_OK;
+    BD_HIP(hipEventSynchronize(ev1));
+    BD_HIP(hipEventElapsedTime(&last_ms, ev0, ev1));
+    timing_pending = false;
+    return BD_OK;
+}
+
+}  // namespace bd
+
+// ---------------------------------------------------------------------------
+// bd_batch
+// ---------------------------------------------------------------------------
+
+struct bd_batch {
+    BatchCore core;
+    std::vector<SceneImage> images;   // host images until the first upload
+    std::vector<uint32_t> sub_steps;  // per scene
+    std::vector<StepParams> params;   // per scene (dt filled per update)
+    std::vector<uint8_t> acc_pending;
+    std::vector<uint64_t> chunk_off;  // byte offset of each scene's chunk
+    std::vector<uint64_t> updates_run;
+    DevBuf<uint8_t> arena;
+    DevBuf<SceneState> states;
+    uint64_t arena_bytes = 0;
+    uint64_t total_points = 0;
+    bool uploaded = false;
+    float last_dt = NAN;
+};
+
+static SceneState *batch_states(bd_batch *b) { return b->states.p; }
+
+static int batch_upload(bd_batch *b, std::string &err) {
+    if (b->uploaded) return BD_OK;
+    int rc = b->core.init_device(err);
+    if (rc) return rc;
+    size_t n = b->images.size();
+    b->chunk_off.resize(n);
+    uint64_t off = 0;
+    for (size_t i = 0; i < n; i++) {
+        b->chunk_off[i] = off;
+        off += (b->images[i].total_bytes + 255) & ~uint64_t(255);
+    }
+    b->arena_bytes = std::max<uint64_t>(off, 256);
+    BD_HIP(b->arena.alloc(b->arena_bytes));
+    BD_HIP(b->states.alloc(std::max<size_t>(n, 1)));
+    std::vector<SceneState> st(n);
+    b->core.descs.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const SceneImage &img = b->images[i];
+        if (img.init_bytes)
+            BD_HIP(hipMemcpyAsync(b->arena.p + b->chunk_off[i], img.bytes.data(), img.init_bytes, hipMemcpyHostToDevice,
+                                  b->core.stream));
+        SceneDesc d = img.desc;
+        d.base = (uint64_t)(uintptr_t)(b->arena.p + b->chunk_off[i]);
+        d.steps = 0xffffffffu;
+        b->core.descs[i] = d;
+        std::memset(&st[i], 0, sizeof(SceneState));
+        st[i].skin = img.skin0;
+    }
+    if (n) BD_HIP(hipMemcpyAsync(b->states.p, st.data(), n * sizeof(SceneState), hipMemcpyHostToDevice, b->core.stream));
+    BD_HIP(hipStreamSynchronize(b->core.stream));
+    b->core.states_ptr = b->states.p;
+    for (auto &img : b->images) { std::vector<uint8_t>().swap(img.bytes); }
+    std::vector<uint32_t> all(n);
+    std::iota(all.begin(), all.end(), 0u);
+    b->core.build_order(all);
+    b->core.descs_dirty = true;
+    b->uploaded = true;
+    return BD_OK;
+}
+
+extern "C" {
+
+bd_batch *bd_batch_new(void) {
+    try {
+        bd_batch *b = new bd_batch();
+        b->core.cfg = bd_batch_config{128, 1024, 2048, 32 * 1024, 144 * 1024, 1, 1};
+        return b;
+    } catch (...) {
+        set_last_error("allocation failed");
+        return nullptr;
+    }
+}
+
+void bd_batch_free(bd_batch *b) {
+    if (!b) return;
+    b->core.release();
+    delete b;
+}
+
+int64_t bd_batch_add_scene(bd_batch *b, const bd_solver *scene) {
+    if (!b || !scene) return BD_ERR_ARG;
+    if (b->uploaded) return fail(BD_ERR_STATE, "scenes must be added before the first bd_batch_update");
+    std::string err;
+    HostScene h;
+    int rc = solver_export(scene, h, err);
+    if (rc) return fail(rc, err);
+    Plan p;
+    rc = build_plan(h, p, err);
+    if (rc) return fail(rc, err);
+    if (!h.polys.empty() || p.N > 16384) return fail(BD_ERR_CAPACITY, "batch scenes hold no soft polygons and at most 16384 bodies");
+    SceneImage img;
+    rc = build_scene_image(h, p, img, err);
+    if (rc) return fail(rc, err);
+    b->images.push_back(std::move(img));
+    b->params.push_back(make_params(h, 0.0f, false));
+    b->acc_pending.push_back(b->images.back().acc_pending ? 1 : 0);
+    b->updates_run.push_back(0);
+    b->total_points += p.N;
+    return (int64_t)b->images.size() - 1;
+}
+
+uint64_t bd_batch_len(const bd_batch *b) { return b ? b->images.size() : 0; }
+uint64_t bd_batch_total_points(const bd_batch *b) { return b ? b->total_points : 0; }
+
+int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps) {
+    if (!b) return BD_ERR_ARG;
+    std::string err;
+    int rc = batch_upload(b, err);
+    if (rc) return fail(rc, err);
+    if (!steps || b->images.empty()) return BD_OK;
+    BatchCore &c = b->core;
+    // descriptors depend on dt and on the pending initial acc
+    bool any_acc = false;
+    for (uint8_t a : b->acc_pending) any_acc |= a != 0;
+    if (!(dt == b->last_dt) || any_acc) {
+        for (size_t i = 0; i < b->images.size(); i++) {
+            StepParams sp = b->params[i];
+            float mult = 1.0f / (float)sp.sub_steps;  // src/solver.rs:107-108
+            sp.delta = dt * mult;
+            set_scene_params(c.descs[i], sp, 0xffffffffu, b->acc_pending[i] != 0);
+        }
+        c.descs_dirty = true;
+        b->last_dt = dt;
+    }
+    uint32_t spl = std::max<uint32_t>(c.cfg.steps_per_launch, 1);
+    if (any_acc) {
+        // the first launch consumes the initial acc; the rest use plain descs
+        rc = c.run(1, 1, err);
+        if (rc) return fail(rc, err);
+        for (size_t i = 0; i < b->images.size(); i++) {
+            c.descs[i].flags &= ~kFlagAcc;
+            b->acc_pending[i] = 0;
+        }
+        c.descs_dirty = true;
+        steps -= 1;
+    }
+    if (steps) {
+        uint32_t full = steps / spl, rem = steps % spl;
+        if (full) { rc = c.run(full, spl, err); if (rc) return fail(rc, err); }
+        if (rem) { rc = c.run(1, rem, err); if (rc) return fail(rc, err); }
+    }
+    for (auto &u : b->updates_run) u += steps + (any_acc ? 1 : 0);
+    return BD_OK;
+}
+
+int32_t bd_batch_sync(bd_batch *b) {
+    if (!b) return BD_ERR_ARG;
+    if (!b->core.stream) return BD_OK;
+    std::string err;
+    if (hipStreamSynchronize(b->core.stream) != hipSuccess) return fail(BD_ERR_DEVICE, "hipStreamSynchronize");
+    int rc = b->core.finish_timing(err);
+    return rc ? fail(rc, err) : BD_OK;
+}
+
+int32_t bd_batch_read_scene(bd_batch *b, int64_t id, bd_solver *out) {
+    if (!b || !out || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
+    std::string err;
+    if (!b->uploaded) {
+        int rc = batch_upload(b, err);
+        if (rc) return fail(rc, err);
+    }
+    const SceneDesc &d = b->core.descs[id];
+    std::vector<float2> pos(d.N), prev(d.N);
+    SceneState st;
+    const uint8_t *chunk = b->arena.p + b->chunk_off[id];
+    if (d.N) {
+        if (hipMemcpyAsync(pos.data(), chunk + d.pos_off, 8ull * d.N, hipMemcpyDeviceToHost, b->core.stream) != hipSuccess ||
+            hipMemcpyAsync(prev.data(), chunk + d.prev_off, 8ull * d.N, hipMemcpyDeviceToHost, b->core.stream) != hipSuccess)
+            return fail(BD_ERR_DEVICE, "scene readback failed");
+    }
+    if (hipMemcpyAsync(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost, b->core.stream) != hipSuccess ||
+        hipStreamSynchronize(b->core.stream) != hipSuccess)
+        return fail(BD_ERR_DEVICE, "scene readback failed");
+    HostScene h;
+    h.p_pos.resize(d.Np); h.p_prev.resize(d.Np);
+    h.c_pos.resize(d.Nc); h.c_prev.resize(d.Nc);
+    for (uint32_t i = 0; i < d.N; i++) {
+        V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
+        if (i < d.Np) { h.p_pos[i] = p; h.p_prev[i] = q; }
+        else { h.c_pos[i - d.Np] = p; h.c_prev[i - d.Np] = q; }
+    }
+    int rc = solver_import_state(out, h, b->updates_run[id], err);
+    if (rc) return fail(rc, err);
+    if (st.err) return fail(BD_ERR_COLLISION, "circle pair schedule failed for this scene");
+    return BD_OK;
+}
+
+int32_t bd_batch_set_config(bd_batch *b, const bd_batch_config *cfg) {
+    if (!b || !cfg) return BD_ERR_ARG;
+    if (variant_for_wg((int)cfg->wg_small) < 0 || variant_for_wg((int)cfg->wg_large) < 0)
+        return fail(BD_ERR_ARG, "workgroup sizes must be 64, 128, 256, 512 or 1024");
+    if (cfg->wg_large * 16ull < 16384 && cfg->split_points < 16384) {
+        // any size is accepted: larger scenes simply loop over their points
+    }
+    bool reorder = cfg->split_points != b->core.cfg.split_points;
+    b->core.cfg = *cfg;
+    if (b->core.cfg.lds_small > scene_max_lds()) b->core.cfg.lds_small = (uint32_t)scene_max_lds();
+    if (b->core.cfg.lds_large > scene_max_lds()) b->core.cfg.lds_large = (uint32_t)scene_max_lds();
+    if (b->core.cfg.steps_per_launch < 1) b->core.cfg.steps_per_launch = 1;
+    b->core.invalidate_graph();
+    if (reorder && b->uploaded) {
+        std::vector<uint32_t> all(b->images.size());
+        std::iota(all.begin(), all.end(), 0u);
+        b->core.build_order(all);
+    }
+    return BD_OK;
+}
+
+int32_t bd_batch_get_config(const bd_batch *b, bd_batch_config *cfg) {
+    if (!b || !cfg) return BD_ERR_ARG;
+    *cfg = b->core.cfg;
+    return BD_OK;
+}
+
+int32_t bd_batch_last_timing(const bd_batch *b, float *ms, uint32_t *launches) {
+    if (!b) return BD_ERR_ARG;
+    std::string err;
+    int rc = const_cast<bd_batch *>(b)->core.finish_timing(err);
+    if (rc) return fail(rc, err);
+    if (ms) *ms = b->core.last_ms;
+    if (launches) *launches = b->core.last_launches;
+    return BD_OK;
+}
+
+int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64_t *rebuilds, uint64_t *err_scenes) {
+    if (!b) return BD_ERR_ARG;
+    size_t n = b->images.size();
+    std::vector<SceneState> st(n);
+    if (b->uploaded && n) {
+        if (hipMemcpy(st.data(), b->states.p, n * sizeof(SceneState), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(BD_ERR_DEVICE, "state readback failed");
+    }
+    uint64_t r = 0, d = 0, u = 0, e = 0;
+    for (const SceneState &s : st) { r += s.rounds; d += s.redos; u += s.rebuilds; e += s.err ? 1 : 0; }
+    if (rounds) *rounds = r;
+    if (redos) *redos = d;
+    if (rebuilds) *rebuilds = u;
+    if (err_scenes) *err_scenes = e;
+    return BD_OK;
+}
+
+int32_t bd_batch_autotune(bd_batch *b, float dt, uint32_t steps) {
+    if (!b) return BD_ERR_ARG;
+    std::string err;
+    int rc = batch_upload(b, err);
+    if (rc) return fail(rc, err);
+    if (b->images.empty()) return 0;
+    BatchCore &c = b->core;
+    // snapshot the whole arena and scene states; restored after tuning
+    DevBuf<uint8_t> snap;
+    DevBuf<SceneState> snap_st;
+    size_t n = b->images.size();
+    if (snap.alloc(b->arena_bytes) != hipSuccess || snap_st.alloc(n) != hipSuccess)
+        return fail(BD_ERR_NOMEM, "no device memory for the autotune snapshot");
+    if (hipMemcpyAsync(snap.p, b->arena.p, b->arena_bytes, hipMemcpyDeviceToDevice, c.stream) != hipSuccess ||
+        hipMemcpyAsync(snap_st.p, b->states.p, n * sizeof(SceneState), hipMemcpyDeviceToDevice, c.stream) != hipSuccess)
+        return fail(BD_ERR_DEVICE, "snapshot copy failed");
+    auto restore = [&]() {
+        (void)hipMemcpyAsync(b->arena.p, snap.p, b->arena_bytes, hipMemcpyDeviceToDevice, c.stream);
+        (void)hipMemcpyAsync(b->states.p, snap_st.p, n * sizeof(SceneState), hipMemcpyDeviceToDevice, c.stream);
+    };
+    std::vector<uint8_t> acc_save = b->acc_pending;
+    std::vector<uint64_t> upd_save = b->updates_run;
+    bd_batch_config best = c.cfg;
+    float best_ms = INFINITY;
+    int tried = 0;
+    auto trial = [&](const bd_batch_config &cf) -> float {
+        if (bd_batch_set_config(b, &cf)) return INFINITY;
+        restore();
+        b->acc_pending = acc_save;
+        float total = 0.0f;
+        for (int rep = 0; rep < 2; rep++) {
+            if (bd_batch_update(b, dt, steps) != BD_OK) return INFINITY;
+            float ms = 0.0f;
+            if (bd_batch_sync(b) != BD_OK) return INFINITY;
+            ms = c.last_ms;
+            if (rep == 1) total = ms;  // second run: graph built, caches warm
+            restore();
+            b->acc_pending = acc_save;
+        }
+        tried++;
+        return total;
+    };
+    const uint32_t wgs[] = {64, 128, 256, 512, 1024};
+    const uint32_t splits[] = {512, 1024, 2048, 4096, 8192};
+    // 1) workgroup sizes and the class split
+    for (uint32_t split : splits)
+        for (uint32_t ws : wgs)
+            for (uint32_t wl : wgs) {
+                if (wl < ws || ws > 256 || wl < 256) continue;
+                bd_batch_config cf = best;
+                cf.split_points = split; cf.wg_small = ws; cf.wg_large = wl;
+                cf.steps_per_launch = 1;
+                float ms = trial(cf);
+                if (ms < best_ms) { best_ms = ms; best = cf; }
+            }
+    // 2) LDS budget (residency) per class
+    const uint32_t lds_opts[] = {16 * 1024, 32 * 1024, 64 * 1024, 144 * 1024};
+    for (uint32_t ls : lds_opts)
+        for (uint32_t ll : lds_opts) {
+            if (ll < 64 * 1024) continue;
+            bd_batch_config cf = best;
+            cf.lds_small = ls; cf.lds_large = ll;
+            float ms = trial(cf);
+            if (ms < best_ms) { best_ms = ms; best = cf; }
+        }
+    // 3) updates per launch
+    for (uint32_t spl : {1u, 2u, 4u, 8u}) {
+        if (spl > steps) break;
+        bd_batch_config cf = best;
+        cf.steps_per_launch = spl;
+        float ms = trial(cf);
+        if (ms < best_ms * 0.98f) { best_ms = ms; best = cf; }
+    }
+    bd_batch_set_config(b, &best);
+    restore();
+    b->acc_pending = acc_save;
+    b->updates_run = upd_save;
+    (void)hipStreamSynchronize(c.stream);
+    b->last_dt = NAN;
+    return tried;
+}
+
+}  // extern "C"

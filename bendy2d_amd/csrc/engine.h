@@ -56,6 +56,13 @@ bool tiled_engine_fits(const HostScene &s, const Plan &p);
 // Thread-local detailed error message used by bd_last_error().
 void set_last_error(const std::string &msg);
 
+}  // namespace bd
+struct bd_solver;
+namespace bd {
+int solver_export(const bd_solver *s, HostScene &out, std::string &err);
+int solver_import_state(bd_solver *s, const HostScene &src, uint64_t updates, std::string &err);
+const HostScene &solver_host(const bd_solver *s);
+
 #define BD_HIP(call)                                                                   \
     do {                                                                               \
         hipError_t e_ = (call);                                                        \

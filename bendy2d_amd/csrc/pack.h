@@ -1,0 +1,89 @@
+// pack.h -- device layout of scenes run by the SCENE engine (one workgroup
+// per scene) and the batched form.
+//
+// Every scene lives in one contiguous chunk of device memory (so scenes can be
+// admitted, evicted and repacked independently):
+//
+//   pos[N] float2 | prev[N] float2 | crad[Nc] f32 | acc0[Nc] float2 |
+//   pinmask[ceil(Np/32)] u32 | layer_off[depth+1] u32 | links[L] LinkS |
+//   collider hdr[ncol] int4 | geo[ncol] float4 | planes[nplanes] float4 |
+//   circle-pair workspace (if Nc >= 2)
+//
+// Scene-local point order: particles [0, Np), circles [Np, N).  Soft
+// polygons are not held by this engine.
+#pragma once
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace bd {
+
+// Scene-local link record (8 bytes).  a, b: point indices < 32768; bit 15 of
+// a / b marks a pinned end.  A link whose a >= Np is a circle link.
+struct LinkS { uint16_t a, b; float t; };
+
+constexpr uint32_t kScenePinBit = 0x8000u;
+constexpr uint32_t kSceneMaxPoints = 32767;
+constexpr uint32_t kCandCap = 64;  // candidate list capacity per scene: kCandCap * Nc entries
+
+// Scene descriptor: read-only in the kernel.
+struct SceneDesc {
+    uint64_t base;               // device address of the scene chunk
+    uint32_t N, Np, Nc, depth;
+    uint32_t L, ncol, nplanes, T;  // T: hash buckets (power of two) for circle pairs
+    uint32_t pos_off, prev_off, crad_off, acc0_off;
+    uint32_t pin_off, loff_off, links_off, hdr_off;
+    uint32_t geo_off, pl_off, ws_off, ws_bytes;
+    uint32_t steps;              // updates to run in this launch
+    uint32_t sub_steps, link_iters;
+    uint32_t flags;              // kFlag*
+    float delta, gex, gey, gx, gy;
+    float lox, loy, hix, hiy;
+    float rmax;
+    uint32_t pad[3];
+};
+
+constexpr uint32_t kFlagPins = 1u;        // some particle is pinned
+constexpr uint32_t kFlagAcc = 2u;         // initial circle acc pending (first substep)
+constexpr uint32_t kFlagLds = 4u;         // positions held in LDS
+
+// Per-scene state that persists between launches (written by the kernel).
+struct SceneState {
+    float skin;          // candidate-list margin of the circle-pair schedule
+    uint32_t list_valid; // candidate list built and still usable
+    uint32_t err;        // nonzero: the circle-pair schedule failed
+    uint32_t rounds;     // schedule rounds executed (statistics)
+    uint32_t redos;      // sweeps redone with a larger skin
+    uint32_t rebuilds;   // candidate lists built
+    uint32_t pairs;      // candidate pairs in the last list
+    uint32_t pad;
+};
+
+// Byte offsets of the circle-pair workspace inside ws (relative to ws_off).
+struct CCLayout {
+    uint32_t ref, save, path, cnt, off, cand, head, key, cell, start, fill, items, bytes;
+};
+
+BD_HD uint32_t align16(uint64_t v) { return (uint32_t)((v + 15) & ~uint64_t(15)); }
+
+BD_HD CCLayout cc_layout(uint32_t Nc, uint32_t T) {
+    CCLayout c{};
+    uint64_t o = 0;
+    c.ref = (uint32_t)o; o = align16(o + 8ull * Nc);
+    c.save = (uint32_t)o; o = align16(o + 8ull * Nc);
+    c.path = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.cnt = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.off = (uint32_t)o; o = align16(o + 4ull * (Nc + 1));
+    c.cand = (uint32_t)o; o = align16(o + 2ull * kCandCap * Nc);
+    c.head = (uint32_t)o; o = align16(o + 8ull * Nc);
+    c.key = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.cell = (uint32_t)o; o = align16(o + 8ull * Nc);
+    c.start = (uint32_t)o; o = align16(o + 4ull * (T + 1));
+    c.fill = (uint32_t)o; o = align16(o + 4ull * T);
+    c.items = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.bytes = (uint32_t)o;
+    return c;
+}
+
+}  // namespace bd

@@ -26,6 +26,7 @@ namespace bd {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
+static const std::string &bd_last_error_str() { return g_last_error; }
 
 StepParams make_params(const HostScene &s, float dt, bool acc_pending) {
     StepParams p;
@@ -95,6 +96,38 @@ static void begin_change(bd_solver *s) {
 }
 
 static uint32_t idx32(uint64_t v) { return v >= 0xffffffffull ? kBadIndex : (uint32_t)v; }
+
+namespace bd {
+
+// Internal accessors for the batched form and the server.
+int solver_export(const bd_solver *s, HostScene &out, std::string &err) {
+    int rc = ensure_host(s);
+    if (rc) { err = bd_last_error_str(); return rc; }
+    out = s->host;
+    return BD_OK;
+}
+
+// Overwrite the body state of `s` with that of `src` (same scene shape), as
+// left by `updates` updates run elsewhere.
+int solver_import_state(bd_solver *s, const HostScene &src, uint64_t updates, std::string &err) {
+    int rc = ensure_host(s);
+    if (rc) { err = bd_last_error_str(); return rc; }
+    HostScene &h = s->host;
+    if (h.p_pos.size() != src.p_pos.size() || h.c_pos.size() != src.c_pos.size() || !h.polys.empty()) {
+        err = "scene shape differs from the batch scene";
+        return BD_ERR_ARG;
+    }
+    h.p_pos = src.p_pos; h.p_prev = src.p_prev;
+    h.c_pos = src.c_pos; h.c_prev = src.c_prev;
+    if (updates) for (V2 &a : h.c_acc) a = V2{0.0f, 0.0f};
+    s->plan_valid = false;  // device copy (if any) is stale now
+    s->dev_newer = false;
+    return BD_OK;
+}
+
+const HostScene &solver_host(const bd_solver *s) { return s->host; }
+
+}  // namespace bd
 
 extern "C" {
 

@@ -11,8 +11,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (BD_ENGINE_AUTO, BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, Bounds, Circle,
-                   Link, Particle, PolygonIn, PolygonInfo, Stats, Vec2, check, lib)
+from ._lib import (BD_ENGINE_AUTO, BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BatchConfig, Bounds,
+                   Circle, Link, Particle, PolygonIn, PolygonInfo, Stats, Vec2, check, lib)
 
 LINK_DTYPE = np.dtype([("a", "<u8"), ("b", "<u8"), ("t", "<f4"), ("pad", "<u4")])
 assert LINK_DTYPE.itemsize == ctypes.sizeof(Link)
@@ -268,3 +268,69 @@ def plan_layers(a, b, n_points):
     order = np.zeros(max(len(a), 1), np.uint64)
     d = check(lib().bd_plan_layers(_ptr(a), _ptr(b), len(a), int(n_points), _ptr(layer), _ptr(order)), "plan_layers")
     return layer[:len(a)], order[:len(a)], d
+
+
+class Batch:
+    """Many independent scenes advanced together (bd_batch_*)."""
+
+    def __init__(self):
+        self._h = lib().bd_batch_new()
+        if not self._h:
+            raise MemoryError("bd_batch_new failed")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bd_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return lib().bd_batch_len(self._h)
+
+    def total_points(self):
+        return lib().bd_batch_total_points(self._h)
+
+    def add_scene(self, solver):
+        return check(lib().bd_batch_add_scene(self._h, solver._h), "batch_add_scene")
+
+    def update(self, dt, steps=1):
+        check(lib().bd_batch_update(self._h, float(dt), int(steps)), "batch_update")
+
+    def sync(self):
+        check(lib().bd_batch_sync(self._h), "batch_sync")
+
+    def read_scene(self, i, solver):
+        check(lib().bd_batch_read_scene(self._h, int(i), solver._h), "batch_read_scene")
+
+    @property
+    def config(self):
+        c = BatchConfig()
+        check(lib().bd_batch_get_config(self._h, ctypes.byref(c)), "batch_get_config")
+        return {k: getattr(c, k) for k, _ in BatchConfig._fields_}
+
+    @config.setter
+    def config(self, d):
+        c = BatchConfig()
+        check(lib().bd_batch_get_config(self._h, ctypes.byref(c)), "batch_get_config")
+        for k, v in d.items():
+            setattr(c, k, int(v))
+        check(lib().bd_batch_set_config(self._h, ctypes.byref(c)), "batch_set_config")
+
+    def autotune(self, dt, steps=2):
+        return check(lib().bd_batch_autotune(self._h, float(dt), int(steps)), "batch_autotune")
+
+    def last_timing(self):
+        ms = ctypes.c_float()
+        n = ctypes.c_uint32()
+        check(lib().bd_batch_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)), "batch_last_timing")
+        return ms.value, n.value
+
+    def cc_stats(self):
+        v = [ctypes.c_uint64() for _ in range(4)]
+        check(lib().bd_batch_cc_stats(self._h, *[ctypes.byref(x) for x in v]), "batch_cc_stats")
+        return dict(zip(("rounds", "redos", "rebuilds", "err_scenes"), [x.value for x in v]))

@@ -252,6 +252,58 @@ int64_t bd_plan_layers(const uint32_t *a, const uint32_t *b, uint64_t n_links, u
                        uint32_t *layer, uint64_t *order);
 
 /* ------------------------------------------------------------------ */
+/* Batched form: many independent scenes in one persistent launch      */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (it has one Solver per world).  A batch
+ * holds copies of independent scenes (no soft polygons, at most 16384 bodies
+ * each) resident in device memory; one persistent kernel launch per size
+ * class advances all of them, one workgroup per scene taken from a work
+ * queue, largest first.  Results per scene are bit-identical to bd_update on
+ * the same scene. */
+typedef struct bd_batch bd_batch;
+
+/* Launch configuration.  Scenes with at most split_points bodies run on
+ * workgroups of wg_small threads, larger ones on wg_large; lds_small /
+ * lds_large bytes of LDS per workgroup decide residency (pos and prev in LDS
+ * when 16 B per body fit, pos alone when 8 B fit, else global memory);
+ * steps_per_launch updates run per kernel launch; use_graph replays the launch
+ * sequence as a captured hipGraph. */
+typedef struct bd_batch_config {
+    uint32_t wg_small, wg_large;
+    uint32_t split_points;
+    uint32_t lds_small, lds_large;
+    uint32_t steps_per_launch;
+    int32_t use_graph;
+} bd_batch_config;
+
+bd_batch *bd_batch_new(void);
+void bd_batch_free(bd_batch *b);
+/* Copy a scene (its state now) into the batch; returns its id (>= 0). */
+int64_t bd_batch_add_scene(bd_batch *b, const bd_solver *scene);
+uint64_t bd_batch_len(const bd_batch *b);
+uint64_t bd_batch_total_points(const bd_batch *b);
+/* Advance every scene by `steps` updates of dt (each update runs the scene's
+ * own sub_steps and link iterations). */
+int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps);
+int32_t bd_batch_sync(bd_batch *b);
+/* Copy scene id's positions back into `out`, a solver holding the same scene
+ * (e.g. the one passed to bd_batch_add_scene, or its clone). */
+int32_t bd_batch_read_scene(bd_batch *b, int64_t id, bd_solver *out);
+int32_t bd_batch_set_config(bd_batch *b, const bd_batch_config *cfg);
+int32_t bd_batch_get_config(const bd_batch *b, bd_batch_config *cfg);
+/* Time candidate configurations on this batch (each for `steps` updates of
+ * dt on a scratch copy of the state, which is restored afterwards) and keep
+ * the fastest.  Returns the number of configurations tried. */
+int32_t bd_batch_autotune(bd_batch *b, float dt, uint32_t steps);
+/* Device time of the last bd_batch_update in milliseconds (HIP events on the
+ * batch's stream around the launches), and the launches it took. */
+int32_t bd_batch_last_timing(const bd_batch *b, float *ms, uint32_t *launches);
+/* Circle-pair schedule counters summed over scenes; err_scenes counts scenes
+ * whose schedule failed (their results are not valid). */
+int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64_t *rebuilds,
+                          uint64_t *err_scenes);
+
+/* ------------------------------------------------------------------ */
 /* Scene generators (synthetic inputs of the benchmark workloads)       */
 /* ------------------------------------------------------------------ */
 /* Link emission order of a grid: COLOUR emits 8 conflict-free classes

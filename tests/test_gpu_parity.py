@@ -8,15 +8,19 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal, first_diff, state, to_oracle
-from bendy2d_amd import BD_ENGINE_GLOBAL, BD_ORDER_PARTICLE, Solver, workloads
+from bendy2d_amd import BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ORDER_PARTICLE, Solver, workloads
+
+ENGINES = [BD_ENGINE_GLOBAL, BD_ENGINE_SCENE]
 
 pytestmark = pytest.mark.gpu
 
 
-def run_pair(s, steps, dt, candidate_mode=False, check_every=None):
+def run_pair(s, steps, dt, candidate_mode=False, check_every=None, engine=None):
     o = to_oracle(s, 32, candidate_mode=candidate_mode)
     for k in range(steps):
         s.update(dt)
+        if engine is not None and k == 0:
+            assert s.engine_used == engine
         assert o.update(dt) == 0
         if check_every and (k + 1) % check_every == 0:
             a, b = state(s), state(o)
@@ -26,7 +30,7 @@ def run_pair(s, steps, dt, candidate_mode=False, check_every=None):
     return s, o
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_cloth_pinned_small(engine):
     s = Solver()
     s.set_engine(engine)
@@ -34,7 +38,7 @@ def test_cloth_pinned_small(engine):
     run_pair(s, 40, dt, check_every=10)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_cloth_per_particle_order(engine):
     s = Solver()
     s.set_engine(engine)
@@ -42,7 +46,7 @@ def test_cloth_per_particle_order(engine):
     run_pair(s, 20, dt)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_reference_defaults(engine):
     # Solver::new defaults: gravity (0, 98.2), bounds (0,0)+(100,100), 1 substep
     s = Solver()
@@ -57,7 +61,7 @@ def test_reference_defaults(engine):
     run_pair(s, 100, 1.0 / 60.0, check_every=25)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_gas_small(engine):
     s = Solver()
     s.set_engine(engine)
@@ -65,7 +69,7 @@ def test_gas_small(engine):
     run_pair(s, 30, dt, check_every=10)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_gas_dense_nan(engine):
     # dense enough that bounds clamp circles onto one corner point: coincident
     # circles normalise a zero vector and yield NaN, as in the reference
@@ -75,7 +79,7 @@ def test_gas_dense_nan(engine):
     run_pair(s, 12, dt, check_every=4)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_circles_links_acc(engine):
     s = Solver()
     s.set_engine(engine)
