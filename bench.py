@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-updates/s of the batched form vs the plain per-scene
+path on one MI355X, plus the 1024x1024 cloth (plain path, HBM roofline) and,
+when built, the serving loop's p50/p99 request latency.
+
+One JSON line on stdout.  `value` is the batched form's aggregate
+particle-updates/s (PU/s) on the 4096-scene softbody workload, inputs resident
+in HBM; a step is one update of every scene (8 substeps each).  A PU is one
+body advanced through one substep.
+
+Algorithmic bytes per substep of a scene (HBM roofline, not MFMA: nothing is a
+dense contraction): 32 B per body (pos + prev read and written), 44 B per link
+solve (12 B record + 2 x 8 B reads + 2 x 8 B writes), plus 32 B per body when
+circle-pair collision is on.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+DT = 1.0 / 60.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_softbody_scenes(n, seed0=1000, max_side=0):
+    from bendy2d_amd import Solver, workloads
+    scenes, pts, links = [], 0, 0
+    for i in range(n):
+        s = Solver()
+        workloads.softbody(s, seed0 + i, max_side=max_side)
+        st = s.stats()
+        pts += st["n_points"]
+        links += st["n_links"]
+        scenes.append(s)
+    return scenes, pts, links
+
+
+def bench_batched(scenes, pts, links, steps, warmup, tune=True):
+    from bendy2d_amd import Batch
+    b = Batch()
+    t0 = time.time()
+    for s in scenes:
+        b.add_scene(s)
+    b.update(DT, 0)  # upload
+    b.sync()
+    log(f"batch: {len(scenes)} scenes, {pts} bodies, {links} links, upload {time.time() - t0:.1f}s")
+    if tune:
+        t0 = time.time()
+        tried = b.autotune(DT, 2)
+        log(f"autotune: {tried} configs in {time.time() - t0:.1f}s -> {b.config}")
+    b.update(DT, warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.update(DT, steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    dev_ms, launches = b.last_timing()
+    cc = b.cc_stats()
+    substeps = 8
+    pu = pts * substeps * steps
+    # bytes per substep: 64 B per body (integrate + collision) + 44 B per link
+    alg_bytes = (64.0 * pts + 44.0 * links) * substeps * steps
+    return {
+        "pu_per_s": pu / wall, "wall_s": wall, "device_ms": dev_ms, "launches": launches,
+        "alg_bytes": alg_bytes, "pu": pu, "config": b.config, "cc": cc, "batch": b,
+    }
+
+
+def bench_plain_per_scene(scenes, pts, updates=1):
+    """The drop-in bd_update called on each scene in turn (AUTO engine)."""
+    for s in scenes:          # warm: plan + upload + first launch
+        s.update(DT)
+    for s in scenes:
+        s.sync()
+    t0 = time.perf_counter()
+    for _ in range(updates):
+        for s in scenes:
+            s.update(DT)
+    for s in scenes:
+        s.sync()
+    wall = time.perf_counter() - t0
+    return {"pu_per_s": pts * 8 * updates / wall, "wall_s": wall}
+
+
+def bench_cloth(n, steps, warmup):
+    from bendy2d_amd import Solver, workloads
+    s = Solver()
+    dt, _ = workloads.cloth_large(s, n=n)
+    s.update(dt)
+    for _ in range(warmup):
+        s.update(dt)
+    s.sync()
+    st = s.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.update(dt)
+    s.sync()
+    wall = time.perf_counter() - t0
+    N, L = st["n_points"], st["n_links"]
+    sub, iters = 4, 8
+    pu = N * sub * steps
+    alg = (32.0 * N + 44.0 * L * iters) * sub * steps
+    return {"pu_per_s": pu / wall, "ms_per_step": 1e3 * wall / steps, "alg_GBps": alg / wall / 1e9,
+            "frac": alg / wall / HBM_PEAK, "engine": s.engine_used, "bodies": N, "links": L,
+            "bytes_per_pu": alg / pu}
+
+
+def cpu_baseline(seconds=12.0, seed0=1000):
+    """The f32 oracle (single thread) on the batched workload's scenes, one
+    update per scene in scene order until `seconds` of CPU time elapse."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from helpers import to_oracle
+    from bendy2d_amd import Solver, workloads
+    pu, busy, n = 0, 0.0, 0
+    while busy < seconds and n < 4096:
+        s = Solver()
+        workloads.softbody(s, seed0 + n)
+        o = to_oracle(s, 32, candidate_mode=True)
+        npts = s.stats()["n_points"]
+        t0 = time.process_time()
+        o.update(DT)
+        busy += time.process_time() - t0
+        pu += npts * 8
+        n += 1
+    return {"value": pu / busy, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off), 1 thread: the first {n} scenes of the "
+                      f"4096-scene workload, 1 update (8 substeps) each, {busy:.1f} s CPU"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scenes", type=int, default=4096)
+    ap.add_argument("--cloth", type=int, default=1024)
+    ap.add_argument("--plain-sample", type=int, default=0, help="scenes for the plain path (0 = all)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box of this run has one MI355X; --gpus {args.gpus} is not run. "
+              "Scenes are independent, so N GPUs would each take 1/N of them (no exchange).", file=sys.stderr)
+        sys.exit(2)
+    from bendy2d_amd import lib
+    if lib().bd_device_count() < 1:
+        log("no HIP device visible")
+        sys.exit(1)
+
+    t0 = time.time()
+    scenes, pts, links = build_softbody_scenes(args.scenes)
+    log(f"built {args.scenes} scenes in {time.time() - t0:.1f}s")
+    bat = bench_batched(scenes, pts, links, args.steps, args.warmup, tune=not args.no_tune)
+    log(f"batched: {bat['pu_per_s']:.3e} PU/s, {bat['wall_s'] * 1e3 / args.steps:.3f} ms/step, cc {bat['cc']}")
+
+    sample = scenes if not args.plain_sample else scenes[::max(1, len(scenes) // args.plain_sample)]
+    spts = sum(s.stats()["n_points"] for s in sample)
+    plain = bench_plain_per_scene(sample, spts)
+    log(f"plain per-scene: {plain['pu_per_s']:.3e} PU/s over {len(sample)} scenes")
+
+    cloth = bench_cloth(args.cloth, 10, 2) if args.cloth else None
+    if cloth:
+        log(f"cloth {args.cloth}^2: {cloth}")
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"cpu: {cpu}")
+
+    launches = max(bat["launches"], 1)
+    graph_launches = max(1, launches // 3) if bat["config"]["use_graph"] else launches
+    per_launch_s = bat["device_ms"] / 1e3 / graph_launches
+    achieved = bat["alg_bytes"] / graph_launches / per_launch_s
+    out = {
+        "metric": "particle-updates/s batched vs plain (1 GPU) + p50/p99 request latency",
+        "value": bat["pu_per_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": bat["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.scenes} independent softbody scenes (grid of circles r=0.35, side log-uniform "
+                        "4..128, structural+shear circle links, 1-8 static circle/convex-polygon colliders, "
+                        "circle-pair collision on), 8 substeps and 1 link sweep per update, batched persistent "
+                        "launch replayed as a hipGraph",
+            "scenes": args.scenes, "bodies": pts, "links": links, "substeps": 8,
+            "launch": {k: v for k, v in bat["config"].items()},
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "kernel": "k_scenes (both size classes, concurrent) per graph launch",
+        },
+        "cpu_baseline": cpu,
+        "plain_per_scene": {"value": plain["pu_per_s"], "scenes": len(sample),
+                            "path": "bd_update per scene (drop-in), AUTO engine"},
+        "batched_vs_plain": bat["pu_per_s"] / plain["pu_per_s"],
+        "cloth_1024": cloth,
+        "serving": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -19,6 +19,7 @@ ARCH = os.environ.get("BENDY2D_ARCH", "gfx950")
 
 SOURCES = [
     "scene.cpp",
+    "tiled_plan.cpp",
     "scenes.cpp",
     "solver.cpp",
     "batch.cpp",

@@ -47,7 +47,7 @@ public:
 
 Engine *make_global_engine(hipStream_t stream);
 Engine *make_scene_engine(hipStream_t stream);
-Engine *make_tiled_engine(hipStream_t stream);
+Engine *make_tiled_engine(hipStream_t stream, uint32_t tile_points, uint32_t wg);
 
 // True when the SCENE engine (one workgroup, LDS-resident) can run this plan.
 bool scene_engine_fits(const HostScene &s, const Plan &p);
@@ -80,6 +80,11 @@ struct DevBuf {
     DevBuf() {}
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);

@@ -21,6 +21,7 @@
 #include "../../include/bendy2d.h"
 #include "common.h"
 #include "engine.h"
+#include "bodies.h"
 #include "kernels.h"
 
 namespace bd {
@@ -162,14 +163,6 @@ __global__ void __launch_bounds__(256) k_poly_pairs(float2 *pos, const uint2 *__
 }
 
 // Fused per-body phase: EXT4 static colliders -> bounds -> Verlet.
-struct BodyParams {
-    float dt, gex, gey, gx, gy;
-    float lox, loy, hix, hiy;
-    uint32_t N, Np, Nc;
-    uint32_t first_acc;
-    uint32_t ncol, nplanes;
-};
-
 constexpr int kMaxStagedColliders = 256;
 constexpr int kMaxStagedPlanes = 1024;
 
@@ -209,25 +202,7 @@ __global__ void __launch_bounds__(256) k_bodies(float2 *__restrict__ pos, float2
             if (P.first_acc && integrate) { float2 a = acc0[u - P.Np]; ax = a.x + P.gx; ay = a.y + P.gy; }
         }
         float2 p = pos[u], q = prev[u];
-        for (uint32_t k = 0; k < P.ncol; k++) {
-            int4 h = hdr[k];
-            if (h.x == 0) {
-                float4 g = geo[k];
-                static_circle(p.x, p.y, r, g.x, g.y, g.z);
-            } else {
-                float smax = 0.0f; int arg = -1;
-                for (int e = 0; e < h.z; e++) {
-                    float4 w = pl[h.y + e];
-                    float sd = (w.x * p.x + w.y * p.y) - w.z;
-                    if (arg < 0 || sd > smax) { smax = sd; arg = e; }
-                }
-                if (smax < r) {
-                    float4 w = pl[h.y + arg];
-                    float push = r - smax;
-                    p.x = p.x + w.x * push; p.y = p.y + w.y * push;
-                }
-            }
-        }
+        static_colliders(p, r, P.ncol, hdr, geo, pl);
         if (circle) {
             circle_bounds_axis(p.x, q.x, P.lox, P.hix, r);
             circle_bounds_axis(p.y, q.y, P.loy, P.hiy, r);

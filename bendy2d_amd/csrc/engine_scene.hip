@@ -28,6 +28,7 @@
 #include "../../include/bendy2d.h"
 #include "common.h"
 #include "engine.h"
+#include "bodies.h"
 #include "kernels.h"
 #include "pack.h"
 #include "scene_pack.h"
@@ -380,26 +381,7 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
                 if (!(pins && j < Np && ((pinmask[j >> 5] >> (j & 31)) & 1u))) {
                     float2 p = P[j];
                     float r = j >= Np ? crad[j - Np] : 0.0f;
-                    for (uint32_t c = 0; c < d.ncol; c++) {
-                        int4 h = hdr[c];
-                        if (h.x == 0) {
-                            float4 g = geo[c];
-                            static_circle(p.x, p.y, r, g.x, g.y, g.z);
-                        } else {
-                            float smax = 0.0f;
-                            int arg = -1;
-                            for (int e = 0; e < h.z; e++) {
-                                float4 w = pl[h.y + e];
-                                float sd = (w.x * p.x + w.y * p.y) - w.z;
-                                if (arg < 0 || sd > smax) { smax = sd; arg = e; }
-                            }
-                            if (smax < r) {
-                                float4 w = pl[h.y + arg];
-                                float push = r - smax;
-                                p.x = p.x + w.x * push; p.y = p.y + w.y * push;
-                            }
-                        }
-                    }
+                    static_colliders(p, r, d.ncol, hdr, geo, pl);
                     float2 q = Q[j];
                     float ax = d.gex, ay = d.gey;
                     if (j >= Np) {

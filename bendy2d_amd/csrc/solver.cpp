@@ -19,6 +19,7 @@
 #include "../../include/bendy2d.h"
 #include "engine.h"
 #include "scene.h"
+#include "tiled_plan.h"
 
 using namespace bd;
 
@@ -56,6 +57,7 @@ struct bd_solver {
     std::unique_ptr<Engine> eng;
     int32_t engine_pref = BD_ENGINE_AUTO;
     int32_t engine_used = 0;
+    uint32_t tile_points = 4096, tile_wg = 1024;
     hipStream_t stream = nullptr;
     int device = -1;
     Stats stats;
@@ -517,6 +519,30 @@ int32_t bd_ext_set_engine(bd_solver *s, int32_t engine) {
 
 int32_t bd_ext_get_engine_used(const bd_solver *s) { return s ? s->engine_used : 0; }
 
+int32_t bd_ext_set_tiling(bd_solver *s, uint32_t tile_points, uint32_t workgroup) {
+    if (!s || tile_points < 64 || tile_points > 16384) return BD_ERR_ARG;
+    if (workgroup != 256 && workgroup != 512 && workgroup != 1024) return BD_ERR_ARG;
+    begin_change(s);
+    s->tile_points = tile_points;
+    s->tile_wg = workgroup;
+    return BD_OK;
+}
+
+int64_t bd_ext_tiled_selfcheck(const bd_solver *s, uint32_t tile_points, uint32_t region_cap, uint32_t *passes) {
+    if (!s) return BD_ERR_ARG;
+    int rc = ensure_host(s);
+    if (rc) return rc;
+    Plan p;
+    std::string err;
+    rc = build_plan(s->host, p, err);
+    if (rc) return fail(rc, err);
+    TilePlan tp;
+    rc = build_tile_plan(s->host, p, tile_points, region_cap, tp, err);
+    if (rc) return fail(rc, err);
+    if (passes) *passes = (uint32_t)tp.passes.size();
+    return (int64_t)tile_plan_selfcheck(s->host, p, tp);
+}
+
 int64_t bd_plan_layers(const uint32_t *a, const uint32_t *b, uint64_t n_links, uint64_t n_points, uint32_t *layer,
                        uint64_t *order) {
     if ((!a || !b || !layer) && n_links) return BD_ERR_ARG;
@@ -555,13 +581,17 @@ int32_t bd_update(bd_solver *s, float dt) {
             return fail(BD_ERR_CAPACITY, "scene does not fit the SCENE engine");
         if (want == BD_ENGINE_TILED && !tiled_engine_fits(s->host, s->plan))
             return fail(BD_ERR_CAPACITY, "link graph does not suit the TILED engine");
-        if (!s->eng || s->eng->kind() != want) {
-            s->eng.reset();
-            if (want == BD_ENGINE_SCENE) s->eng.reset(make_scene_engine(s->stream));
-            else if (want == BD_ENGINE_TILED) s->eng.reset(make_tiled_engine(s->stream));
-            else s->eng.reset(make_global_engine(s->stream));
-        }
+        s->eng.reset();
+        if (want == BD_ENGINE_SCENE) s->eng.reset(make_scene_engine(s->stream));
+        else if (want == BD_ENGINE_TILED) s->eng.reset(make_tiled_engine(s->stream, s->tile_points, s->tile_wg));
+        else s->eng.reset(make_global_engine(s->stream));
         rc = s->eng->upload(s->host, s->plan, err);
+        if (rc == BD_ERR_CAPACITY && want == BD_ENGINE_TILED && s->engine_pref == BD_ENGINE_AUTO) {
+            // the link graph does not tile within the LDS budget: one launch per layer
+            want = BD_ENGINE_GLOBAL;
+            s->eng.reset(make_global_engine(s->stream));
+            rc = s->eng->upload(s->host, s->plan, err);
+        }
         if (rc) return fail(rc, err);
         s->plan_valid = true;
         s->acc_pending = s->host.any_initial_acc();

@@ -184,6 +184,16 @@ class Solver:
     def set_engine(self, engine):
         check(lib().bd_ext_set_engine(self._h, int(engine)), "set_engine")
 
+    def set_tiling(self, tile_points=4096, workgroup=1024):
+        check(lib().bd_ext_set_tiling(self._h, int(tile_points), int(workgroup)), "set_tiling")
+
+    def tiled_selfcheck(self, tile_points=4096, region_cap=16384):
+        """Host check of the TILED schedule: (mismatching values, passes per sweep)."""
+        passes = ctypes.c_uint32()
+        bad = check(lib().bd_ext_tiled_selfcheck(self._h, int(tile_points), int(region_cap), ctypes.byref(passes)),
+                    "tiled_selfcheck")
+        return bad, passes.value
+
     @property
     def engine_used(self):
         return lib().bd_ext_get_engine_used(self._h)

@@ -222,6 +222,15 @@ int32_t bd_ext_set_pinned_range(bd_solver *s, uint64_t first, const uint8_t *pin
 #define BD_ENGINE_TILED 3
 int32_t bd_ext_set_engine(bd_solver *s, int32_t engine);
 int32_t bd_ext_get_engine_used(const bd_solver *s);
+/* TILED engine shape: interior points per tile (64..16384) and workgroup
+ * size (256, 512 or 1024).  Defaults 4096 and 1024. */
+int32_t bd_ext_set_tiling(bd_solver *s, uint32_t tile_points, uint32_t workgroup);
+/* Host check of the TILED schedule for this scene: builds the tile plan
+ * (region_cap points per tile region at most), runs one link sweep of it on
+ * the host with the engine's arithmetic and compares it with the sequential
+ * sweep.  Returns the number of differing values (0 = exact) or a negative
+ * BD_ERR_*; passes receives the passes per sweep. */
+int64_t bd_ext_tiled_selfcheck(const bd_solver *s, uint32_t tile_points, uint32_t region_cap, uint32_t *passes);
 
 /* Wait for all queued device work of this solver. */
 int32_t bd_ext_sync(bd_solver *s);

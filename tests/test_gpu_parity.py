@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal, first_diff, state, to_oracle
-from bendy2d_amd import BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ORDER_PARTICLE, Solver, workloads
+from bendy2d_amd import BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BD_ORDER_PARTICLE, Solver, workloads
 
 ENGINES = [BD_ENGINE_GLOBAL, BD_ENGINE_SCENE]
 
@@ -118,3 +118,38 @@ def test_mixed_scene(engine):
     s.set_sub_steps(3)
     s.set_link_iterations(2)
     run_pair(s, 60, 1.0 / 60.0, check_every=20)
+
+
+@pytest.mark.parametrize("tile,wg", [(4096, 512), (1024, 256), (2048, 1024)])
+def test_tiled_cloth(tile, wg):
+    # 160x160 jittered pinned cloth, 4 substeps, 8 sweeps, plus a static circle
+    s = Solver()
+    s.set_engine(BD_ENGINE_TILED)
+    s.set_tiling(tile, wg)
+    dt, _ = workloads.cloth_large(s, n=160)
+    s.add_static_circle((80.0, -200.0), 30.0)
+    run_pair(s, 30, dt, check_every=10, engine=BD_ENGINE_TILED)
+
+
+def test_tiled_deep_layers():
+    # per-particle link order: hundreds of layers, several passes per sweep
+    s = Solver()
+    s.set_engine(BD_ENGINE_TILED)
+    s.set_tiling(1024, 256)
+    dt, _ = workloads.cloth_large(s, n=140, order=BD_ORDER_PARTICLE, sub_steps=2, link_iters=2)
+    run_pair(s, 10, dt, engine=BD_ENGINE_TILED)
+
+
+def test_auto_engine_choice():
+    small = Solver()
+    workloads.cloth_pinned(small, n=32)
+    small.update(1 / 60)
+    assert small.engine_used == BD_ENGINE_SCENE
+    big = Solver()
+    workloads.cloth_large(big, n=200)
+    big.update(1 / 60)
+    assert big.engine_used == BD_ENGINE_TILED
+    poly = Solver()
+    poly.add_polygon_circle(5.0, (50.0, 50.0), 8)
+    poly.update(1 / 60)
+    assert poly.engine_used == BD_ENGINE_GLOBAL
