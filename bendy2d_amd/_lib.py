@@ -5,7 +5,7 @@ raises, and every solver call runs the library's HIP code.
 """
 import ctypes
 import os
-from ctypes import (POINTER, Structure, c_char_p, c_float, c_int32, c_int64,
+from ctypes import (POINTER, Structure, c_char_p, c_float, c_int32, c_int64,  # noqa: F401
                     c_uint8, c_uint32, c_uint64, c_void_p)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -70,6 +70,24 @@ class BatchConfig(Structure):
     _fields_ = [("wg_small", c_uint32), ("wg_large", c_uint32), ("split_points", c_uint32),
                 ("lds_small", c_uint32), ("lds_large", c_uint32), ("steps_per_launch", c_uint32),
                 ("use_graph", c_int32)]
+
+
+class ServerConfig(Structure):
+    _fields_ = [("memory_budget", c_uint64), ("steps_per_launch", c_uint32), ("max_active", c_uint32),
+                ("launch", BatchConfig)]
+
+
+class RequestDone(Structure):
+    _fields_ = [("request_id", c_int64), ("scene_id", c_int64), ("steps", c_uint32), ("status", c_int32),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("particle_updates", c_uint64)]
+
+
+class ServerStats(Structure):
+    _fields_ = [("resident_scenes", c_uint64), ("known_scenes", c_uint64), ("bytes_used", c_uint64),
+                ("bytes_budget", c_uint64), ("launches", c_uint64), ("particle_updates", c_uint64),
+                ("requests_done", c_uint64), ("requests_pending", c_uint64), ("evictions", c_uint64),
+                ("readmissions", c_uint64), ("waiting_admission", c_uint64), ("busy_s", ctypes.c_double)]
 
 
 BD_ORDER_COLOUR = 0
@@ -160,6 +178,15 @@ _SIGS = [
     ("bd_batch_autotune", c_int32, [_S, c_float, c_uint32]),
     ("bd_batch_last_timing", c_int32, [_S, POINTER(c_float), POINTER(c_uint32)]),
     ("bd_batch_cc_stats", c_int32, [_S, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
+    ("bd_server_new", _S, [POINTER(ServerConfig)]),
+    ("bd_server_free", None, [_S]),
+    ("bd_server_now", ctypes.c_double, [_S]),
+    ("bd_server_submit_create", c_int64, [_S, _S, c_uint32, c_float, POINTER(c_int64)]),
+    ("bd_server_submit_continue", c_int64, [_S, c_int64, c_uint32, c_float]),
+    ("bd_server_poll", c_int32, [_S, POINTER(RequestDone), c_int32]),
+    ("bd_server_wait_idle", c_int32, [_S, ctypes.c_double]),
+    ("bd_server_read_scene", c_int32, [_S, c_int64, _S]),
+    ("bd_server_get_stats", c_int32, [_S, POINTER(ServerStats)]),
     ("bd_plan_layers", c_int64, [c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p]),
 ]
 

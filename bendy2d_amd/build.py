@@ -23,6 +23,7 @@ SOURCES = [
     "scenes.cpp",
     "solver.cpp",
     "batch.cpp",
+    "server.cpp",
     "util.hip",
     "engine_global.hip",
     "engine_scene.hip",

@@ -228,6 +228,7 @@ static int batch_upload(bd_batch *b, std::string &err) {
         SceneDesc d = img.desc;
         d.base = (uint64_t)(uintptr_t)(b->arena.p + b->chunk_off[i]);
         d.steps = 0xffffffffu;
+        d.state_slot = (uint32_t)i;
         b->core.descs[i] = d;
         std::memset(&st[i], 0, sizeof(SceneState));
         st[i].skin = img.skin0;

@@ -449,11 +449,11 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
         const uint32_t steps = d.steps < launch_steps ? d.steps : launch_steps;
         if (steps == 0 || d.N == 0) continue;
         if (16ull * d.N <= lds_bytes)
-            run_scene<WG, 2>(d, steps, states + si, s_lds, &s_col, s_red, s_u32, s_flag);
+            run_scene<WG, 2>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, s_flag);
         else if (8ull * d.N <= lds_bytes)
-            run_scene<WG, 1>(d, steps, states + si, s_lds, &s_col, s_red, s_u32, s_flag);
+            run_scene<WG, 1>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, s_flag);
         else
-            run_scene<WG, 0>(d, steps, states + si, s_lds, &s_col, s_red, s_u32, s_flag);
+            run_scene<WG, 0>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, s_flag);
     }
 }
 

@@ -41,7 +41,8 @@ struct SceneDesc {
     float delta, gex, gey, gx, gy;
     float lox, loy, hix, hiy;
     float rmax;
-    uint32_t pad[3];
+    uint32_t state_slot;         // index of this scene's SceneState
+    uint32_t pad[2];
 };
 
 constexpr uint32_t kFlagPins = 1u;        // some particle is pinned

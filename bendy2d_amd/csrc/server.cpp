@@ -1,0 +1,450 @@
+// server.cpp -- serving: continuous batching of scene-advance requests over
+// scenes kept resident in device memory (include/bendy2d.h, bd_server_*).
+//
+// A scheduler thread repeats: take newly submitted requests; pick the scenes
+// whose oldest request is pending (FIFO by submit time); make each resident
+// (admitting it into the device arena, evicting least recently used idle
+// scenes to host memory when the budget is full); pack them into one compact
+// descriptor table, each with min(remaining, steps_per_launch) updates; run one
+// persistent batched launch; retire requests whose steps are done.  Requests
+// submitted during a launch join the next one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bendy2d.h"
+#include "batch.h"
+#include "engine.h"
+#include "scene_pack.h"
+
+using namespace bd;
+
+namespace {
+
+// First-fit allocator over one device allocation, 256-byte granules.
+struct Arena {
+    uint8_t *base = nullptr;
+    uint64_t size = 0, used = 0;
+    std::map<uint64_t, uint64_t> free_;  // offset -> bytes
+    bool alloc(uint64_t bytes, uint64_t &off) {
+        bytes = (bytes + 255) & ~uint64_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                off = it->first;
+                uint64_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        bytes = (bytes + 255) & ~uint64_t(255);
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Request {
+    int64_t id, scene;
+    uint32_t steps, remaining;
+    float dt;
+    double t_submit, t_start = -1.0;
+};
+
+struct ServedScene {
+    int64_t id;
+    SceneDesc desc{};               // base / steps / delta / state_slot set per launch
+    std::vector<uint8_t> host;      // chunk init bytes while not resident
+    uint64_t init_bytes = 0, total_bytes = 0;
+    float skin0 = 0.05f;
+    StepParams params{};
+    bool acc_pending = false;
+    bool resident = false;
+    uint64_t off = 0;
+    uint32_t slot = 0;
+    SceneState saved{};
+    std::deque<Request> queue;
+    uint64_t last_used = 0;
+    uint64_t updates_run = 0;
+};
+
+}  // namespace
+
+struct bd_server {
+    bd_server_config cfg{};
+    BatchCore core;
+    Arena arena;
+    DevBuf<SceneState> slots;
+    std::vector<uint32_t> free_slots;
+    std::chrono::steady_clock::time_point t0;
+
+    std::mutex mu;                  // incoming, done, scenes map structure
+    std::condition_variable cv, idle_cv;
+    std::deque<Request> incoming;
+    std::deque<bd_request_done> done;
+    std::unordered_map<int64_t, std::unique_ptr<ServedScene>> scenes;
+    std::mutex launch_mu;           // held by the scheduler while a launch and its bookkeeping run
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    int64_t next_request = 1, next_scene = 1;
+    uint64_t pending = 0;           // requests submitted and not finished
+    std::string fatal;
+
+    // statistics (scheduler thread; read under mu)
+    uint64_t launches = 0, pu = 0, n_done = 0, evictions = 0, readmissions = 0, waiting = 0;
+    double busy_s = 0.0;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static int fail(int code, const std::string &msg) {
+    set_last_error(msg);
+    return code;
+}
+
+static constexpr uint32_t kSlotCap = 1u << 20;
+
+// Device work of the scheduler thread.  Returns false on a fatal error.
+static bool admit(bd_server *S, ServedScene &sc, const std::vector<ServedScene *> &keep, std::string &err) {
+    if (sc.resident) return true;
+    uint64_t off;
+    while (!S->arena.alloc(sc.total_bytes, off)) {
+        // evict the least recently used idle resident scene (the map can grow
+        // under submit, so it is scanned under the server mutex; queues and
+        // residency are only changed by this thread)
+        ServedScene *victim = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto &kv : S->scenes) {
+                ServedScene *v = kv.second.get();
+                if (!v->resident || !v->queue.empty()) continue;
+                if (std::find(keep.begin(), keep.end(), v) != keep.end()) continue;
+                if (!victim || v->last_used < victim->last_used) victim = v;
+            }
+        }
+        if (!victim) return false;  // waits for space
+        victim->host.resize(victim->init_bytes);
+        if (hipMemcpyAsync(victim->host.data(), S->arena.base + victim->off, victim->init_bytes, hipMemcpyDeviceToHost,
+                           S->core.stream) != hipSuccess ||
+            hipMemcpyAsync(&victim->saved, S->slots.p + victim->slot, sizeof(SceneState), hipMemcpyDeviceToHost,
+                           S->core.stream) != hipSuccess ||
+            hipStreamSynchronize(S->core.stream) != hipSuccess) {
+            err = "eviction copy failed";
+            return false;
+        }
+        S->arena.release(victim->off, victim->total_bytes);
+        S->free_slots.push_back(victim->slot);
+        victim->resident = false;
+        S->evictions++;
+    }
+    if (S->free_slots.empty()) {
+        S->arena.release(off, sc.total_bytes);
+        return false;
+    }
+    sc.slot = S->free_slots.back();
+    S->free_slots.pop_back();
+    sc.off = off;
+    if (hipMemcpyAsync(S->arena.base + off, sc.host.data(), sc.init_bytes, hipMemcpyHostToDevice, S->core.stream) != hipSuccess ||
+        hipMemcpyAsync(S->slots.p + sc.slot, &sc.saved, sizeof(SceneState), hipMemcpyHostToDevice, S->core.stream) != hipSuccess ||
+        hipStreamSynchronize(S->core.stream) != hipSuccess) {
+        err = "admission copy failed";
+        return false;
+    }
+    if (sc.updates_run) S->readmissions++;
+    std::vector<uint8_t>().swap(sc.host);
+    sc.resident = true;
+    return true;
+}
+
+static void scheduler(bd_server *S) {
+    std::string err;
+    const uint32_t spl = std::max<uint32_t>(S->cfg.steps_per_launch, 1);
+    for (;;) {
+        std::vector<ServedScene *> run;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait_for(lk, std::chrono::milliseconds(2),
+                           [&] { return S->stop.load() || !S->incoming.empty() || S->pending > 0; });
+            if (S->stop.load()) return;
+            while (!S->incoming.empty()) {
+                Request r = S->incoming.front();
+                S->incoming.pop_front();
+                S->scenes[r.scene]->queue.push_back(r);
+            }
+            // scenes with a pending request, oldest request first
+            for (auto &kv : S->scenes)
+                if (!kv.second->queue.empty()) run.push_back(kv.second.get());
+        }
+        if (run.empty()) continue;
+        std::sort(run.begin(), run.end(), [](const ServedScene *a, const ServedScene *b) {
+            return a->queue.front().t_submit < b->queue.front().t_submit ||
+                   (a->queue.front().t_submit == b->queue.front().t_submit && a->id < b->id);
+        });
+        if (S->cfg.max_active && run.size() > S->cfg.max_active) run.resize(S->cfg.max_active);
+        std::lock_guard<std::mutex> lg(S->launch_mu);
+        std::vector<ServedScene *> launch;
+        uint64_t waiting = 0;
+        for (ServedScene *sc : run) {
+            if (admit(S, *sc, run, err)) launch.push_back(sc);
+            else if (!err.empty()) { S->fatal = err; return; }
+            else waiting++;
+        }
+        S->waiting = waiting;
+        if (launch.empty()) continue;
+        // compact descriptor table for this launch
+        const double t_start = S->now();
+        BatchCore &c = S->core;
+        c.descs.resize(launch.size());
+        for (size_t i = 0; i < launch.size(); i++) {
+            ServedScene *sc = launch[i];
+            Request &rq = sc->queue.front();
+            if (rq.t_start < 0) rq.t_start = t_start;
+            StepParams sp = sc->params;
+            float mult = 1.0f / (float)sp.sub_steps;  // src/solver.rs:107-108
+            sp.delta = rq.dt * mult;
+            SceneDesc d = sc->desc;
+            d.base = (uint64_t)(uintptr_t)(S->arena.base + sc->off);
+            d.state_slot = sc->slot;
+            set_scene_params(d, sp, std::min(rq.remaining, spl), sc->acc_pending);
+            c.descs[i] = d;
+        }
+        std::vector<uint32_t> all(launch.size());
+        std::iota(all.begin(), all.end(), 0u);
+        c.build_order(all);
+        c.descs_dirty = true;
+        int rc = c.run(1, spl, err);
+        if (rc == BD_OK) rc = c.finish_timing(err);
+        if (rc != BD_OK) { S->fatal = err; return; }
+        const double t_done = S->now();
+        std::vector<bd_request_done> fin;
+        uint64_t pu = 0;
+        for (ServedScene *sc : launch) {
+            Request &rq = sc->queue.front();
+            uint32_t ran = std::min(rq.remaining, spl);
+            rq.remaining -= ran;
+            sc->updates_run += ran;
+            sc->acc_pending = false;
+            sc->last_used = S->launches;
+            pu += (uint64_t)sc->desc.N * sc->params.sub_steps * ran;
+            if (rq.remaining == 0) {
+                bd_request_done d{};
+                d.request_id = rq.id; d.scene_id = sc->id; d.steps = rq.steps; d.status = BD_OK;
+                d.t_submit = rq.t_submit; d.t_start = rq.t_start; d.t_done = t_done;
+                d.particle_updates = (uint64_t)sc->desc.N * sc->params.sub_steps * rq.steps;
+                fin.push_back(d);
+                sc->queue.pop_front();
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->launches++;
+            S->pu += pu;
+            S->busy_s += c.last_ms * 1e-3;
+            for (auto &d : fin) S->done.push_back(d);
+            S->n_done += fin.size();
+            S->pending -= fin.size();
+        }
+        if (!fin.empty()) S->idle_cv.notify_all();
+    }
+}
+
+extern "C" {
+
+bd_server *bd_server_new(const bd_server_config *cfg) {
+    if (!cfg || cfg->memory_budget < 4096) { set_last_error("memory budget below 4 KiB"); return nullptr; }
+    std::unique_ptr<bd_server> S(new bd_server());
+    S->cfg = *cfg;
+    S->core.cfg = cfg->launch;
+    if (!S->core.cfg.wg_small) S->core.cfg = bd_batch_config{128, 1024, 2048, 32 * 1024, 144 * 1024, 1, 0};
+    S->core.cfg.use_graph = 0;
+    S->core.cfg.steps_per_launch = std::max<uint32_t>(cfg->steps_per_launch, 1);
+    if (variant_for_wg((int)S->core.cfg.wg_small) < 0 || variant_for_wg((int)S->core.cfg.wg_large) < 0) {
+        set_last_error("workgroup sizes must be 64, 128, 256, 512 or 1024");
+        return nullptr;
+    }
+    std::string err;
+    if (S->core.init_device(err)) { set_last_error(err); return nullptr; }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { set_last_error("hipMemGetInfo failed"); return nullptr; }
+    uint64_t budget = std::min<uint64_t>(cfg->memory_budget, (uint64_t)(free_b * 0.9));
+    budget &= ~uint64_t(255);
+    if (hipMalloc((void **)&S->arena.base, budget) != hipSuccess) { set_last_error("arena allocation failed"); return nullptr; }
+    S->arena.size = budget;
+    S->arena.free_[0] = budget;
+    if (S->slots.alloc(kSlotCap) != hipSuccess) { set_last_error("state slots allocation failed"); return nullptr; }
+    S->core.states_ptr = S->slots.p;
+    S->free_slots.reserve(kSlotCap);
+    for (uint32_t i = kSlotCap; i-- > 0;) S->free_slots.push_back(i);
+    S->t0 = std::chrono::steady_clock::now();
+    bd_server *raw = S.release();
+    raw->worker = std::thread(scheduler, raw);
+    return raw;
+}
+
+void bd_server_free(bd_server *S) {
+    if (!S) return;
+    S->stop.store(true);
+    S->cv.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    if (S->core.stream) (void)hipStreamSynchronize(S->core.stream);
+    if (S->arena.base) (void)hipFree(S->arena.base);
+    delete S;
+}
+
+double bd_server_now(const bd_server *S) { return S ? S->now() : 0.0; }
+
+int64_t bd_server_submit_create(bd_server *S, const bd_solver *scene, uint32_t steps, float dt, int64_t *scene_id) {
+    if (!S || !scene) return BD_ERR_ARG;
+    std::string err;
+    HostScene h;
+    int rc = solver_export(scene, h, err);
+    if (rc) return fail(rc, err);
+    Plan p;
+    rc = build_plan(h, p, err);
+    if (rc) return fail(rc, err);
+    if (!h.polys.empty() || p.N > 16384) return fail(BD_ERR_CAPACITY, "served scenes hold no soft polygons and at most 16384 bodies");
+    SceneImage img;
+    rc = build_scene_image(h, p, img, err);
+    if (rc) return fail(rc, err);
+    std::unique_ptr<ServedScene> sc(new ServedScene());
+    sc->desc = img.desc;
+    sc->host = std::move(img.bytes);
+    sc->init_bytes = img.init_bytes;
+    sc->total_bytes = img.total_bytes;
+    sc->skin0 = img.skin0;
+    sc->saved.skin = img.skin0;
+    sc->params = make_params(h, dt, img.acc_pending);
+    sc->acc_pending = img.acc_pending;
+    if (sc->total_bytes > S->arena.size) return fail(BD_ERR_BUDGET, "scene larger than the memory budget");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->fatal.empty()) return fail(BD_ERR_DEVICE, S->fatal);
+    int64_t sid = S->next_scene++, rid = S->next_request++;
+    sc->id = sid;
+    S->scenes[sid] = std::move(sc);
+    S->incoming.push_back(Request{rid, sid, steps, steps, dt, S->now()});
+    S->pending++;
+    if (scene_id) *scene_id = sid;
+    S->cv.notify_one();
+    return rid;
+}
+
+int64_t bd_server_submit_continue(bd_server *S, int64_t scene_id, uint32_t steps, float dt) {
+    if (!S) return BD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->fatal.empty()) return fail(BD_ERR_DEVICE, S->fatal);
+    if (!S->scenes.count(scene_id)) return fail(BD_ERR_INDEX, "unknown scene id");
+    int64_t rid = S->next_request++;
+    S->incoming.push_back(Request{rid, scene_id, steps, steps, dt, S->now()});
+    S->pending++;
+    S->cv.notify_one();
+    return rid;
+}
+
+int32_t bd_server_poll(bd_server *S, bd_request_done *out, int32_t max) {
+    if (!S || (!out && max > 0)) return BD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    int32_t n = 0;
+    while (n < max && !S->done.empty()) {
+        out[n++] = S->done.front();
+        S->done.pop_front();
+    }
+    return n;
+}
+
+int32_t bd_server_wait_idle(bd_server *S, double timeout_s) {
+    if (!S) return BD_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    bool ok = S->idle_cv.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                                  [&] { return S->pending == 0 || !S->fatal.empty(); });
+    if (!S->fatal.empty()) return fail(BD_ERR_DEVICE, S->fatal);
+    return ok ? BD_OK : fail(BD_ERR_STATE, "requests still pending at timeout");
+}
+
+int32_t bd_server_read_scene(bd_server *S, int64_t scene_id, bd_solver *out) {
+    if (!S || !out) return BD_ERR_ARG;
+    std::lock_guard<std::mutex> lg(S->launch_mu);  // no launch in flight
+    ServedScene *sc;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->scenes.find(scene_id);
+        if (it == S->scenes.end()) return fail(BD_ERR_INDEX, "unknown scene id");
+        sc = it->second.get();
+    }
+    const SceneDesc &d = sc->desc;
+    std::vector<uint8_t> buf(sc->init_bytes);
+    SceneState st = sc->saved;
+    if (sc->resident) {
+        if (hipMemcpyAsync(buf.data(), S->arena.base + sc->off, sc->init_bytes, hipMemcpyDeviceToHost, S->core.stream) != hipSuccess ||
+            hipMemcpyAsync(&st, S->slots.p + sc->slot, sizeof(st), hipMemcpyDeviceToHost, S->core.stream) != hipSuccess ||
+            hipStreamSynchronize(S->core.stream) != hipSuccess)
+            return fail(BD_ERR_DEVICE, "scene readback failed");
+    } else {
+        buf = sc->host;
+    }
+    const float2 *pos = (const float2 *)(buf.data() + d.pos_off);
+    const float2 *prev = (const float2 *)(buf.data() + d.prev_off);
+    HostScene h;
+    h.p_pos.resize(d.Np); h.p_prev.resize(d.Np);
+    h.c_pos.resize(d.Nc); h.c_prev.resize(d.Nc);
+    for (uint32_t i = 0; i < d.N; i++) {
+        V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
+        if (i < d.Np) { h.p_pos[i] = p; h.p_prev[i] = q; }
+        else { h.c_pos[i - d.Np] = p; h.c_prev[i - d.Np] = q; }
+    }
+    std::string err;
+    int rc = solver_import_state(out, h, sc->updates_run, err);
+    if (rc) return fail(rc, err);
+    if (st.err) return fail(BD_ERR_COLLISION, "circle pair schedule failed for this scene");
+    return BD_OK;
+}
+
+int32_t bd_server_get_stats(bd_server *S, bd_server_stats *out) {
+    if (!S || !out) return BD_ERR_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::memset(out, 0, sizeof(*out));
+    for (auto &kv : S->scenes) out->resident_scenes += kv.second->resident ? 1 : 0;
+    out->known_scenes = S->scenes.size();
+    out->bytes_used = S->arena.used;
+    out->bytes_budget = S->arena.size;
+    out->launches = S->launches;
+    out->particle_updates = S->pu;
+    out->requests_done = S->n_done;
+    out->requests_pending = S->pending;
+    out->evictions = S->evictions;
+    out->readmissions = S->readmissions;
+    out->waiting_admission = S->waiting;
+    out->busy_s = S->busy_s;
+    return BD_OK;
+}
+
+}  // extern "C"

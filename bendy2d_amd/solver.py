@@ -12,7 +12,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BD_ENGINE_AUTO, BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BatchConfig, Bounds,
-                   Circle, Link, Particle, PolygonIn, PolygonInfo, Stats, Vec2, check, lib)
+                   Circle, Link, Particle, PolygonIn, PolygonInfo, RequestDone, ServerConfig, ServerStats, Stats,
+                   Vec2, check, lib)
 
 LINK_DTYPE = np.dtype([("a", "<u8"), ("b", "<u8"), ("t", "<f4"), ("pad", "<u4")])
 assert LINK_DTYPE.itemsize == ctypes.sizeof(Link)
@@ -344,3 +345,63 @@ class Batch:
         v = [ctypes.c_uint64() for _ in range(4)]
         check(lib().bd_batch_cc_stats(self._h, *[ctypes.byref(x) for x in v]), "batch_cc_stats")
         return dict(zip(("rounds", "redos", "rebuilds", "err_scenes"), [x.value for x in v]))
+
+
+class Server:
+    """Continuous-batching server of scene-advance requests (bd_server_*)."""
+
+    def __init__(self, memory_budget=64 << 30, steps_per_launch=8, max_active=0, launch=None):
+        cfg = ServerConfig()
+        cfg.memory_budget = int(memory_budget)
+        cfg.steps_per_launch = int(steps_per_launch)
+        cfg.max_active = int(max_active)
+        if launch:
+            for k, v in launch.items():
+                setattr(cfg.launch, k, int(v))
+        self._h = lib().bd_server_new(ctypes.byref(cfg))
+        if not self._h:
+            raise RuntimeError("bd_server_new failed: " + lib().bd_last_error().decode())
+        self._buf = (RequestDone * 4096)()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bd_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self):
+        return lib().bd_server_now(self._h)
+
+    def create(self, solver, steps, dt):
+        sid = ctypes.c_int64()
+        rid = check(lib().bd_server_submit_create(self._h, solver._h, int(steps), float(dt), ctypes.byref(sid)),
+                    "server_submit_create")
+        return rid, sid.value
+
+    def advance(self, scene_id, steps, dt):
+        return check(lib().bd_server_submit_continue(self._h, int(scene_id), int(steps), float(dt)),
+                     "server_submit_continue")
+
+    def poll(self):
+        out = []
+        while True:
+            n = check(lib().bd_server_poll(self._h, self._buf, len(self._buf)), "server_poll")
+            out += [{k: getattr(self._buf[i], k) for k, _ in RequestDone._fields_} for i in range(n)]
+            if n < len(self._buf):
+                return out
+
+    def wait_idle(self, timeout=60.0):
+        check(lib().bd_server_wait_idle(self._h, float(timeout)), "server_wait_idle")
+
+    def read_scene(self, scene_id, solver):
+        check(lib().bd_server_read_scene(self._h, int(scene_id), solver._h), "server_read_scene")
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().bd_server_get_stats(self._h, ctypes.byref(s)), "server_get_stats")
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}

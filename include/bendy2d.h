@@ -313,6 +313,60 @@ int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64
                           uint64_t *err_scenes);
 
 /* ------------------------------------------------------------------ */
+/* Serving: continuous batching of scene-advance requests              */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference.  A server keeps scenes resident in device
+ * memory within a byte budget and runs a scheduler thread: every launch
+ * advances each scene that has a pending request by up to steps_per_launch
+ * updates, in one persistent batched launch over all of them; requests that
+ * arrive meanwhile join the next launch.  A scene's requests run in arrival
+ * order.  When the budget is full, idle scenes (no pending request) are
+ * evicted to host memory, least recently used first, and brought back when a
+ * request names them; a new scene that cannot fit waits.  Results are
+ * bit-identical to bd_update on the same scene, alone or in any mix. */
+typedef struct bd_server bd_server;
+
+typedef struct bd_server_config {
+    uint64_t memory_budget;     /* bytes of device memory for resident scenes */
+    uint32_t steps_per_launch;  /* updates per launch for each active scene */
+    uint32_t max_active;        /* scenes per launch at most (0 = no limit) */
+    bd_batch_config launch;     /* launch shape (wg sizes, split, LDS); steps_per_launch and use_graph ignored */
+} bd_server_config;
+
+/* A finished request.  Times are seconds on the server's monotonic clock
+ * (bd_server_now). */
+typedef struct bd_request_done {
+    int64_t request_id, scene_id;
+    uint32_t steps;
+    int32_t status;             /* BD_OK or a negative BD_ERR_* */
+    double t_submit, t_start, t_done;
+    uint64_t particle_updates;  /* bodies x substeps x steps */
+} bd_request_done;
+
+typedef struct bd_server_stats {
+    uint64_t resident_scenes, known_scenes, bytes_used, bytes_budget;
+    uint64_t launches, particle_updates, requests_done, requests_pending;
+    uint64_t evictions, readmissions, waiting_admission;
+    double busy_s;              /* device time of all launches */
+} bd_server_stats;
+
+bd_server *bd_server_new(const bd_server_config *cfg);
+void bd_server_free(bd_server *srv);
+double bd_server_now(const bd_server *srv);
+/* New scene (a copy of `scene`) advanced by `steps` updates of dt.  Returns
+ * the request id; *scene_id receives the id later requests use. */
+int64_t bd_server_submit_create(bd_server *srv, const bd_solver *scene, uint32_t steps, float dt, int64_t *scene_id);
+/* Advance a known scene by `steps` more updates of dt. */
+int64_t bd_server_submit_continue(bd_server *srv, int64_t scene_id, uint32_t steps, float dt);
+/* Move up to max finished requests into out; returns how many. */
+int32_t bd_server_poll(bd_server *srv, bd_request_done *out, int32_t max);
+/* Block until no request is pending (or timeout_s passes: BD_ERR_STATE). */
+int32_t bd_server_wait_idle(bd_server *srv, double timeout_s);
+/* Copy a scene's current state into `out` (a solver holding the same scene). */
+int32_t bd_server_read_scene(bd_server *srv, int64_t scene_id, bd_solver *out);
+int32_t bd_server_get_stats(bd_server *srv, bd_server_stats *out);
+
+/* ------------------------------------------------------------------ */
 /* Scene generators (synthetic inputs of the benchmark workloads)       */
 /* ------------------------------------------------------------------ */
 /* Link emission order of a grid: COLOUR emits 8 conflict-free classes

@@ -1,0 +1,105 @@
+"""Serving on the GPU: each request's result is bit-identical to the f32
+oracle running that scene alone, whatever else shares the launches, also when
+the memory budget forces scenes out to host memory and back."""
+import numpy as np
+import pytest
+
+from helpers import bits_equal, first_diff, state, to_oracle
+from bendy2d_amd import Server, Solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+
+
+def scene(seed, max_side=14):
+    s = Solver()
+    workloads.softbody(s, seed, max_side=max_side)
+    return s
+
+
+def oracle_state(s, steps_dts):
+    o = to_oracle(s, 32)
+    for steps, dt in steps_dts:
+        for _ in range(steps):
+            assert o.update(dt) == 0
+    return state(o)
+
+
+def test_mix_matches_oracle():
+    rng = np.random.default_rng(3)
+    srv = Server(memory_budget=1 << 30, steps_per_launch=4)
+    scenes, ids, history = [], [], []
+    for i in range(12):
+        s = scene(200 + i)
+        k = int(rng.integers(1, 12))
+        _, sid = srv.create(s, k, DT)
+        scenes.append(s); ids.append(sid); history.append([(k, DT)])
+    # continues, some with another dt, interleaved with creates
+    for j in range(20):
+        i = int(rng.integers(0, len(ids)))
+        k = int(rng.integers(1, 9))
+        dt = DT if j % 3 else 1.0 / 120.0
+        srv.advance(ids[i], k, dt)
+        history[i].append((k, dt))
+    srv.wait_idle(120)
+    done = srv.poll()
+    assert len(done) == 32 and all(d["status"] == 0 for d in done)
+    assert all(d["t_done"] >= d["t_start"] >= d["t_submit"] for d in done)
+    for i, s in enumerate(scenes):
+        want = oracle_state(s, history[i])
+        srv.read_scene(ids[i], s)
+        got = state(s)
+        assert bits_equal(got, want), f"scene {i}: {first_diff(got, want)}"
+    st = srv.stats()
+    assert st["requests_done"] == 32 and st["requests_pending"] == 0
+    srv.close()
+
+
+def test_eviction_and_readmission():
+    # a budget of about three scene chunks: scenes are evicted to host memory
+    # between their requests and brought back, without changing results
+    s0 = scene(300, max_side=8)
+    one = Server(memory_budget=1 << 30)
+    one.create(s0.clone(), 1, DT)
+    one.wait_idle(60)
+    chunk = one.stats()["bytes_used"]
+    one.close()
+    srv = Server(memory_budget=3 * chunk + 1024, steps_per_launch=2)
+    scenes = [scene(300, max_side=8) for i in range(6)]  # same size: exactly three fit
+    ids = []
+    for s in scenes:
+        ids.append(srv.create(s, 3, DT)[1])
+        srv.wait_idle(60)
+    for sid in ids:
+        srv.advance(sid, 2, DT)
+        srv.wait_idle(60)
+    st = srv.stats()
+    assert st["evictions"] > 0 and st["readmissions"] > 0
+    for i, s in enumerate(scenes):
+        want = oracle_state(s, [(3, DT), (2, DT)])
+        srv.read_scene(ids[i], s)
+        assert bits_equal(state(s), want), f"scene {i}"
+    srv.close()
+
+
+def test_alone_equals_in_mix():
+    target = scene(777, max_side=24)
+    alone = Server(memory_budget=1 << 30, steps_per_launch=3)
+    sid = alone.create(target.clone(), 10, DT)[1]
+    alone.wait_idle(60)
+    a = target.clone()
+    alone.read_scene(sid, a)
+    alone.close()
+    mix = Server(memory_budget=1 << 30, steps_per_launch=3)
+    others = [scene(900 + i, max_side=32) for i in range(30)]
+    for o in others[:15]:
+        mix.create(o, 7, DT)
+    sid2 = mix.create(target.clone(), 10, DT)[1]
+    for o in others[15:]:
+        mix.create(o, 13, DT)
+    mix.wait_idle(60)
+    b = target.clone()
+    mix.read_scene(sid2, b)
+    mix.close()
+    assert bits_equal(state(a), state(b))
