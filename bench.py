@@ -113,6 +113,61 @@ def bench_cloth(n, steps, warmup):
             "bytes_per_pu": alg / pu}
 
 
+def bench_serving(seconds, rate, launch_cfg, steps_per_launch=8, budget=64 << 30, seed=4):
+    """Poisson stream of requests: half create a new softbody scene (generated
+    as in the batched workload, seeded by request number), half continue a
+    uniformly chosen known scene; each advances its scene K ~ U{10..600}
+    updates.  Arrivals stop after `seconds`; the server then drains."""
+    import threading
+    import numpy as np
+    from bendy2d_amd import Server, Solver, workloads
+    rng = np.random.default_rng(seed)
+    n = int(rate * seconds * 1.2) + 16
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < seconds]
+    kinds = rng.random(len(arrivals)) < 0.5       # True: create
+    ks = rng.integers(10, 601, len(arrivals))
+    picks = rng.random(len(arrivals))
+    srv = Server(memory_budget=budget, steps_per_launch=steps_per_launch,
+                 launch={k: launch_cfg[k] for k in ("wg_small", "wg_large", "split_points", "lds_small", "lds_large")})
+    known = []
+    lag = []
+    t0 = time.perf_counter()
+    for i, ta in enumerate(arrivals):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        lag.append(time.perf_counter() - t0 - ta)
+        if kinds[i] or not known:
+            s = Solver()
+            workloads.softbody(s, 500000 + i)
+            known.append(srv.create(s, int(ks[i]), DT)[1])
+            s.close()
+        else:
+            srv.advance(known[int(picks[i] * len(known))], int(ks[i]), DT)
+    t_arrive_end = time.perf_counter() - t0
+    srv.wait_idle(600)
+    t_end = time.perf_counter() - t0
+    done = srv.poll()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([d["t_done"] - d["t_submit"] for d in done])
+    pu = sum(d["particle_updates"] for d in done)
+    bad = sum(1 for d in done if d["status"] != 0)
+    return {
+        "requests": len(done), "rate_req_per_s": rate, "arrival_seconds": seconds,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3),
+        "pu_per_s": pu / t_end, "pu_per_s_device_busy": st["particle_updates"] / max(st["busy_s"], 1e-9),
+        "device_busy_frac": st["busy_s"] / t_end, "drain_s": t_end - t_arrive_end,
+        "launches": st["launches"], "resident_scenes": st["resident_scenes"], "bytes_used": st["bytes_used"],
+        "evictions": st["evictions"], "failed": bad,
+        "driver_lag_ms_p99": float(np.percentile(np.array(lag), 99) * 1e3),
+        "steps_per_launch": steps_per_launch,
+    }
+
+
 def cpu_baseline(seconds=12.0, seed0=1000):
     """The f32 oracle (single thread) on the batched workload's scenes, one
     update per scene in scene order until `seconds` of CPU time elapse."""
@@ -145,6 +200,8 @@ def main():
     ap.add_argument("--cloth", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=0, help="scenes for the plain path (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -170,6 +227,15 @@ def main():
     cloth = bench_cloth(args.cloth, 10, 2) if args.cloth else None
     if cloth:
         log(f"cloth {args.cloth}^2: {cloth}")
+
+    serving = None
+    if args.serve_seconds > 0:
+        # free the batched workload's device memory first
+        bat["batch"].close()
+        for s in scenes:
+            s.close()
+        serving = bench_serving(args.serve_seconds, args.serve_rate, bat["config"])
+        log(f"serving: {serving}")
 
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
@@ -210,7 +276,7 @@ def main():
                             "path": "bd_update per scene (drop-in), AUTO engine"},
         "batched_vs_plain": bat["pu_per_s"] / plain["pu_per_s"],
         "cloth_1024": cloth,
-        "serving": None,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -426,6 +426,23 @@ int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64
     return BD_OK;
 }
 
+int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *err, uint32_t *rounds,
+                              uint32_t *redos, uint32_t *rebuilds, uint32_t *pairs) {
+    if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
+    SceneState st{};
+    if (b->uploaded) {
+        if (hipMemcpy(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(BD_ERR_DEVICE, "state readback failed");
+    }
+    if (skin) *skin = st.skin;
+    if (err) *err = st.err;
+    if (rounds) *rounds = st.rounds;
+    if (redos) *redos = st.redos;
+    if (rebuilds) *rebuilds = st.rebuilds;
+    if (pairs) *pairs = st.pairs;
+    return BD_OK;
+}
+
 int32_t bd_batch_autotune(bd_batch *b, float dt, uint32_t steps) {
     if (!b) return BD_ERR_ARG;
     std::string err;

@@ -93,44 +93,33 @@ __device__ void wg_scan_global(uint32_t *a, uint32_t n, uint32_t *s_wave) {
 // circle at a non-finite position takes part in no pair (every comparison
 // with it is false), so it cannot invalidate the list; one whose list-build
 // position was non-finite but is finite now forces a rebuild.
-__device__ __forceinline__ float moved_bound(float2 p, float2 r0, float path) {
+//
+// The bound is taken relative to a common shift u of the whole scene: for a
+// pair, |(p_i - r_i) - (p_j - r_j)| <= |p_i - r_i - u| + |p_j - r_j - u| for
+// any fixed u, so bulk motion (a falling body) does not count.
+__device__ __forceinline__ float moved_bound(float2 p, float2 r0, float2 u, float path) {
     if (!(fabsf(p.x) <= 3.0e38f) || !(fabsf(p.y) <= 3.0e38f)) return 0.0f;
     if (!(fabsf(r0.x) <= 3.0e38f) || !(fabsf(r0.y) <= 3.0e38f)) return INFINITY;
-    float dx = p.x - r0.x, dy = p.y - r0.y;
+    float dx = (p.x - r0.x) - u.x, dy = (p.y - r0.y) - u.y;
     float v = sqrtf(dx * dx + dy * dy) + path;
     return v == v ? v : INFINITY;
 }
 
-// Candidates of circle c from the hashed cells around it (3x3 cells of size
-// >= 2 r_max + skin); writes them to out when given, returns the count.
-__device__ __noinline__ uint32_t gather_candidates(uint32_t c, const float2 *__restrict__ ref,
-                                                   const float *__restrict__ crad, const int2 *__restrict__ cell,
-                                                   const uint32_t *__restrict__ start,
-                                                   const uint32_t *__restrict__ items, uint32_t tmask, float skin,
-                                                   uint16_t *out) {
-    float2 p = ref[c];
-    float rc = crad[c];
-    int2 ce = cell[c];
-    uint32_t n = 0;
-    for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-            int cx = ce.x + dx, cy = ce.y + dy;
-            uint32_t kk = cell_hash(cx, cy) & tmask;
-            for (uint32_t s = start[kk], s1 = start[kk + 1]; s < s1; s++) {
-                uint32_t o = items[s];
-                if (o == c) continue;
-                int2 co = cell[o];
-                if (co.x != cx || co.y != cy) continue;
-                float2 po = ref[o];
-                float ex = p.x - po.x, ey = p.y - po.y;
-                float lim = (rc + crad[o]) + skin;
-                if (ex * ex + ey * ey < lim * lim) {
-                    if (out) out[n] = (uint16_t)o;
-                    n++;
-                }
-            }
-        }
-    return n;
+// Sum of (x, y, count) over the workgroup (every thread gets the result).
+template <int WG>
+__device__ __forceinline__ float3 wg_sum3(float3 v, float *s_red3) {
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o, 64);
+        v.y += __shfl_xor(v.y, o, 64);
+        v.z += __shfl_xor(v.z, o, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) { s_red3[3 * wave] = v.x; s_red3[3 * wave + 1] = v.y; s_red3[3 * wave + 2] = v.z; }
+    __syncthreads();
+    float3 t = make_float3(0.0f, 0.0f, 0.0f);
+    for (int w = 0; w < WG / 64; w++) { t.x += s_red3[3 * w]; t.y += s_red3[3 * w + 1]; t.z += s_red3[3 * w + 2]; }
+    return t;
 }
 
 __device__ __forceinline__ int scell(float x, double inv_h) {
@@ -142,9 +131,237 @@ __device__ __forceinline__ int scell(float x, double inv_h) {
 
 // MODE 2: pos and prev in LDS; MODE 1: pos in LDS, prev in global memory;
 // MODE 0: both in global memory (the scene chunk).
+struct CCWork {
+    float2 *ref, *save;
+    float *path, *skin, *need;
+    uint32_t *cnt, *off;
+    uint16_t *cand;
+    uint32_t *head, *stamp, *wl, *key;
+    int2 *cell;
+    uint32_t *start, *fill, *items;
+};
+
+struct CCShared {
+    float red3[3 * 16];      // workgroup sums
+    uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
+    uint32_t flag;
+};
+
+// Candidates of circle c: every o != c with |ref_c - ref_o| < r_c + r_o +
+// skin_c + skin_o.  Cells have size h and every circle searches the
+// (2R+1)^2 cells around its own, R covering the largest possible pair
+// distance 2 (r_max + skin_max), so each pair is found from both ends.
+// Writes the candidates to out when given; returns the count.
+__device__ __forceinline__ uint32_t gather_candidates(uint32_t c, const CCWork &w, const float *crad, int R,
+                                                      uint32_t tmask, uint16_t *out) {
+    const float2 p = w.ref[c];
+    const float rc = crad[c], sc = w.skin[c];
+    const int2 ce = w.cell[c];
+    uint32_t n = 0;
+    for (int dy = -R; dy <= R; dy++)
+        for (int dx = -R; dx <= R; dx++) {
+            int cx = ce.x + dx, cy = ce.y + dy;
+            uint32_t kk = cell_hash(cx, cy) & tmask;
+            for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
+                uint32_t o = w.items[s];
+                if (o == c) continue;
+                int2 co = w.cell[o];
+                if (co.x != cx || co.y != cy) continue;
+                float2 po = w.ref[o];
+                float ex = p.x - po.x, ey = p.y - po.y;
+                float lim = (rc + crad[o]) + (sc + w.skin[o]);
+                if (ex * ex + ey * ey < lim * lim) {
+                    if (out) out[n] = (uint16_t)o;
+                    n++;
+                }
+            }
+        }
+    return n;
+}
+
+// One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
+// candidate pairs run in rounds; pair (c, o), c < o, is ready when o is the
+// smallest unprocessed candidate of c and c that of o, so every earlier pair
+// (in the reference's (i, j) order) sharing c or o has run.  Ready pairs share
+// no circle.  After a round only circles whose head moved can complete a new
+// ready pair, so each round looks at those alone (a worklist).  The sweep is
+// valid when no circle moved more than 0.8 of its skin since the list was
+// built (relative to the scene's mean shift): a pair not on the list started
+// further apart than the sum of the two radii and skins, so it cannot have
+// overlapped.  Otherwise the sweep is undone and rerun with larger skins for
+// the circles that moved.
+// Returns the new error state (0 = fine).
+template <int WG>
+__device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
+                                 const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, float &sbase,
+                                 bool &list_valid, uint32_t &n_rounds, uint32_t &n_redos, uint32_t &n_rebuilds,
+                                 uint32_t &n_pairs) {
+    const int tid = threadIdx.x;
+    const uint32_t tmask = T - 1;
+    for (int attempt = 0;; attempt++) {
+        if (attempt > 12) return 2u;
+        bool rebuild = !list_valid;
+        float2 u = make_float2(0.0f, 0.0f);
+        if (!rebuild) {
+            // drift since the list was built, relative to the mean shift u
+            float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                float2 p = P[Np + c], r0 = w.ref[c];
+                float dx = p.x - r0.x, dy = p.y - r0.y;
+                if (fabsf(dx) <= 3.0e38f && fabsf(dy) <= 3.0e38f) { acc.x += dx; acc.y += dy; acc.z += 1.0f; }
+            }
+            acc = wg_sum3<WG>(acc, sh->red3);
+            if (acc.z > 0.0f) u = make_float2(acc.x / acc.z, acc.y / acc.z);
+            float bad = 0.0f;
+            for (uint32_t c = tid; c < Nc; c += WG)
+                if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.4f * w.skin[c]) bad = 1.0f;
+            rebuild = wg_max<WG>(bad, s_red) > 0.0f;
+        }
+        if (rebuild) {
+            n_rebuilds++;
+            // per-circle skins: the base, or what the circle's last sweep pushed it
+            float smax = 0.0f;
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                float sk = fmaxf(sbase, w.need[c]);
+                w.skin[c] = sk;
+                smax = fmaxf(smax, sk);
+            }
+            smax = wg_max<WG>(smax, s_red);
+            // cell size: the base pair distance, grown so the reach stays <= 4 cells
+            const double reach = 2.0 * ((double)rmax + (double)smax);
+            const double h = fmax(2.0 * ((double)rmax + (double)sbase), reach / 4.0) * 1.001 + 1e-30;
+            const int R = (int)ceil(reach / h);
+            const double inv_h = 1.0 / h;
+            for (uint32_t b = tid; b <= T; b += WG) w.start[b] = 0u;
+            for (uint32_t b = tid; b < T; b += WG) w.fill[b] = 0u;
+            __syncthreads();
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                float2 p = P[Np + c];
+                w.ref[c] = p;
+                int cx = scell(p.x, inv_h), cy = scell(p.y, inv_h);
+                w.cell[c] = make_int2(cx, cy);
+                uint32_t kk = cell_hash(cx, cy) & tmask;
+                w.key[c] = kk;
+                atomicAdd(&w.start[kk], 1u);
+            }
+            __syncthreads();
+            wg_scan_global<WG>(w.start, T + 1, s_u32);  // bucket k = [start[k], start[k+1])
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                uint32_t kk = w.key[c];
+                w.items[w.start[kk] + atomicAdd(&w.fill[kk], 1u)] = c;
+            }
+            __syncthreads();
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                uint32_t n = gather_candidates(c, w, crad, R, tmask, nullptr);
+                w.cnt[c] = n;
+                w.off[c] = n;
+            }
+            if (tid == 0) w.off[Nc] = 0u;
+            __syncthreads();
+            wg_scan_global<WG>(w.off, Nc + 1, s_u32);
+            const uint32_t total = w.off[Nc];
+            if (total > kCandCap * Nc) {
+                // more candidates than the workspace holds: smaller skins
+                __syncthreads();
+                sbase = 0.7f * sbase;
+                for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = 0.5f * w.need[c];
+                __syncthreads();
+                list_valid = false;
+                if (sbase < 1e-4f * fmaxf(rmax, 1e-3f)) return 1u;
+                continue;
+            }
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                uint16_t *my = w.cand + w.off[c];
+                uint32_t n = gather_candidates(c, w, crad, R, tmask, my);
+                for (uint32_t x = 1; x < n; x++) {  // ascending: the reference's pair order
+                    uint16_t v = my[x];
+                    uint32_t y = x;
+                    while (y > 0 && my[y - 1] > v) { my[y] = my[y - 1]; y--; }
+                    my[y] = v;
+                }
+            }
+            list_valid = true;
+            n_pairs = total / 2;
+            u = make_float2(0.0f, 0.0f);
+        }
+        // ---- sweep: worklist rounds
+        for (uint32_t c = tid; c < Nc; c += WG) {
+            w.save[c] = P[Np + c];
+            w.path[c] = 0.0f;
+            w.head[c] = 0u;
+            w.stamp[c] = 0xffffffffu;
+        }
+        if (tid == 0) { sh->cnt3[0] = 0u; sh->cnt3[1] = 0u; sh->cnt3[2] = 0u; }
+        __syncthreads();
+        uint32_t *wl0 = w.wl, *wl1 = w.wl + (Nc / 2 + 1);
+        for (uint32_t c = tid; c < Nc; c += WG) {
+            if (w.cnt[c] == 0) continue;
+            uint32_t o = w.cand[w.off[c]];
+            if (c < o && w.cand[w.off[o]] == c) wl0[atomicAdd(&sh->cnt3[0], 1u)] = c | (o << 16);
+        }
+        __syncthreads();
+        uint32_t n = sh->cnt3[0];
+        for (uint32_t r = 0; n > 0; r++) {
+            uint32_t *cur = (r & 1) ? wl1 : wl0, *nxt = (r & 1) ? wl0 : wl1;
+            // apply the ready pairs and move both heads
+            for (uint32_t k = tid; k < n; k += WG) {
+                uint32_t pr = cur[k], c = pr & 0xffffu, o = pr >> 16;
+                float2 pi = P[Np + c], pj = P[Np + o];
+                float qi = 0.0f, qj = 0.0f;
+                if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, crad[c], crad[o], qi, qj)) {
+                    P[Np + c] = pi;
+                    P[Np + o] = pj;
+                    w.path[c] = w.path[c] + qi;
+                    w.path[o] = w.path[o] + qj;
+                }
+                w.head[c] = w.head[c] + 1u;
+                w.head[o] = w.head[o] + 1u;
+                w.stamp[c] = r;
+                w.stamp[o] = r;
+            }
+            if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;  // read at the end of round r-1, two barriers ago
+            __syncthreads();
+            // pairs completed by a moved head
+            for (uint32_t k = tid; k < n; k += WG) {
+                uint32_t pr = cur[k];
+                for (int e = 0; e < 2; e++) {
+                    uint32_t x = e ? (pr >> 16) : (pr & 0xffffu);
+                    uint32_t hx = w.head[x];
+                    if (hx >= w.cnt[x]) continue;
+                    uint32_t y = w.cand[w.off[x] + hx];
+                    uint32_t hy = w.head[y];
+                    if (hy >= w.cnt[y] || w.cand[w.off[y] + hy] != x) continue;
+                    if (w.stamp[y] == r && y < x) continue;  // both moved: the smaller one adds it
+                    uint32_t a = x < y ? x : y, b = x < y ? y : x;
+                    nxt[atomicAdd(&sh->cnt3[(r + 1) % 3], 1u)] = a | (b << 16);
+                }
+            }
+            __syncthreads();
+            n = sh->cnt3[(r + 1) % 3];
+            n_rounds++;
+        }
+        // every list must be exhausted (a schedule that stalls is an error)
+        float left = 0.0f, fail = 0.0f;
+        for (uint32_t c = tid; c < Nc; c += WG) {
+            if (w.head[c] != w.cnt[c]) left = 1.0f;
+            float pth = w.path[c];
+            float v = moved_bound(w.save[c], w.ref[c], u, pth);
+            w.need[c] = (pth <= 3.0e38f) ? 2.5f * pth : 0.0f;  // the sweep's own pushes
+            if (!(v <= 0.8f * w.skin[c])) fail = 1.0f;
+        }
+        if (wg_max<WG>(left, s_red) > 0.0f) return 3u;
+        if (wg_max<WG>(fail, s_red) == 0.0f) return 0u;
+        // undo and rebuild with the skins the circles needed
+        for (uint32_t c = tid; c < Nc; c += WG) P[Np + c] = w.save[c];
+        __syncthreads();
+        n_redos++;
+        list_valid = false;
+    }
+}
+
 template <int WG, int MODE>
 __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, float2 *s_lds, ColliderStage *s_col,
-                          float *s_red, uint32_t *s_u32, uint32_t *s_flag) {
+                          float *s_red, uint32_t *s_u32, CCShared *s_cc) {
     char *base = (char *)d.base;
     float2 *gpos = (float2 *)(base + d.pos_off);
     float2 *gprev = (float2 *)(base + d.prev_off);
@@ -180,19 +397,17 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
     // circle-pair workspace
     char *ws = base + d.ws_off;
     const CCLayout cl = cc_layout(Nc, d.T);
-    float2 *ref = (float2 *)(ws + cl.ref);
-    float2 *save = (float2 *)(ws + cl.save);
-    float *path = (float *)(ws + cl.path);
-    uint32_t *cnt = (uint32_t *)(ws + cl.cnt);
-    uint32_t *coff = (uint32_t *)(ws + cl.off);
-    uint16_t *cand = (uint16_t *)(ws + cl.cand);
-    uint32_t *head = (uint32_t *)(ws + cl.head);
-    uint32_t *key = (uint32_t *)(ws + cl.key);
-    int2 *cell = (int2 *)(ws + cl.cell);
-    uint32_t *start = (uint32_t *)(ws + cl.start);
-    uint32_t *fill = (uint32_t *)(ws + cl.fill);
-    uint32_t *items = (uint32_t *)(ws + cl.items);
-    const uint32_t tmask = d.T ? d.T - 1 : 0;
+    CCWork w;
+    w.ref = (float2 *)(ws + cl.ref); w.save = (float2 *)(ws + cl.save);
+    w.path = (float *)(ws + cl.path); w.skin = (float *)(ws + cl.skin); w.need = (float *)(ws + cl.need);
+    w.cnt = (uint32_t *)(ws + cl.cnt); w.off = (uint32_t *)(ws + cl.off); w.cand = (uint16_t *)(ws + cl.cand);
+    w.head = (uint32_t *)(ws + cl.head); w.stamp = (uint32_t *)(ws + cl.stamp); w.wl = (uint32_t *)(ws + cl.wl);
+    w.key = (uint32_t *)(ws + cl.key); w.cell = (int2 *)(ws + cl.cell); w.start = (uint32_t *)(ws + cl.start);
+    w.fill = (uint32_t *)(ws + cl.fill); w.items = (uint32_t *)(ws + cl.items);
+    if (Nc >= 2 && !st->ws_ready) {
+        for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = 0.0f;
+        __syncthreads();
+    }
 
     float skin = st->skin;
     bool list_valid = st->list_valid != 0;
@@ -224,155 +439,9 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
                 }
             }
             // ---- circle pairs
-            if (Nc >= 2 && !err) {
-                for (int attempt = 0;; attempt++) {
-                    // displacement since the list was built
-                    float dmax = 0.0f;
-                    if (list_valid) {
-                        for (uint32_t j = Np + tid; j < N; j += WG) {
-                            {
-                                float2 p = P[j], r0 = ref[j - Np];
-                                dmax = fmaxf(dmax, moved_bound(p, r0, 0.0f));
-                            }
-                        }
-                        dmax = wg_max<WG>(dmax, s_red);
-                    }
-                    if (!list_valid || dmax > 0.25f * skin) {
-                        // ---- build the candidate list
-                        n_rebuilds++;
-                        double inv_h = 1.0 / (((double)(2.0f * d.rmax) + (double)skin) * 1.001 + 1e-30);
-                        for (uint32_t b = tid; b <= d.T; b += WG) start[b] = 0u;
-                        for (uint32_t b = tid; b < d.T; b += WG) fill[b] = 0u;
-                        __syncthreads();
-                        for (uint32_t j = Np + tid; j < N; j += WG) {
-                            {
-                                uint32_t c = j - Np;
-                                float2 p = P[j];
-                                ref[c] = p;
-                                int cx = scell(p.x, inv_h), cy = scell(p.y, inv_h);
-                                cell[c] = make_int2(cx, cy);
-                                uint32_t kk = cell_hash(cx, cy) & tmask;
-                                key[c] = kk;
-                                atomicAdd(&start[kk], 1u);
-                            }
-                        }
-                        __syncthreads();
-                        // counts -> bucket starts; start[T] = Nc, bucket k = [start[k], start[k+1])
-                        wg_scan_global<WG>(start, d.T + 1, s_u32);
-                        for (uint32_t j = Np + tid; j < N; j += WG) {
-                            {
-                                uint32_t c = j - Np;
-                                uint32_t kk = key[c];
-                                uint32_t slot = start[kk] + atomicAdd(&fill[kk], 1u);
-                                items[slot] = c;
-                            }
-                        }
-                        __syncthreads();
-                        // candidates of circle c: every o != c within r_c + r_o + skin of it
-                        // at list build; counted, scanned into offsets, then written and
-                        // sorted ascending (the reference's pair order)
-                        for (uint32_t j = Np + tid; j < N; j += WG) {
-                            uint32_t c = j - Np;
-                            uint32_t n = gather_candidates(c, ref, crad, cell, start, items, tmask, skin, nullptr);
-                            cnt[c] = n;
-                            coff[c] = n;
-                        }
-                        if (tid == 0) coff[Nc] = 0u;
-                        __syncthreads();
-                        wg_scan_global<WG>(coff, Nc + 1, s_u32);
-                        if (coff[Nc] <= kCandCap * Nc) {
-                            for (uint32_t j = Np + tid; j < N; j += WG) {
-                                uint32_t c = j - Np;
-                                uint16_t *my = cand + coff[c];
-                                uint32_t n = gather_candidates(c, ref, crad, cell, start, items, tmask, skin, my);
-                                for (uint32_t x = 1; x < n; x++) {
-                                    uint16_t v = my[x];
-                                    uint32_t y = x;
-                                    while (y > 0 && my[y - 1] > v) { my[y] = my[y - 1]; y--; }
-                                    my[y] = v;
-                                }
-                            }
-                        }
-                        uint32_t total = coff[Nc];
-                        __syncthreads();
-                        if (total > kCandCap * Nc) {
-                            // more candidates than the workspace holds: shrink the skin
-                            list_valid = false;
-                            skin = skin * 0.7f;
-                            if (skin < 1e-4f * fmaxf(d.rmax, 1e-3f) || attempt > 12) { err = 1; break; }
-                            continue;
-                        }
-                        uint32_t pairs = tid == 0 ? total : 0u;
-                        list_valid = true;
-                        n_pairs = pairs;  // per-lane partial; summed below for statistics
-                    }
-                    // ---- sweep
-                    for (uint32_t j = Np + tid; j < N; j += WG) {
-                        {
-                            uint32_t c = j - Np;
-                            save[c] = P[j];
-                            path[c] = 0.0f;
-                            head[c] = 0u;
-                        }
-                    }
-                    if (tid == 0) { s_flag[0] = 0u; s_flag[1] = 0u; s_flag[2] = 0u; }
-                    __syncthreads();
-                    uint32_t *hin = head, *hout = head + Nc;
-                    for (uint32_t r = 0;; r++) {
-                        uint32_t any = 0;
-                        for (uint32_t j = Np + tid; j < N; j += WG) {
-                            {
-                                uint32_t c = j - Np;
-                                uint32_t h = hin[c], nc_ = cnt[c], nh = h;
-                                if (h < nc_) {
-                                    uint32_t o = cand[coff[c] + h];
-                                    uint32_t ho = hin[o];
-                                    if (ho < cnt[o] && cand[coff[o] + ho] == c) {
-                                        if (c < o) {
-                                            float2 pi = P[Np + c], pj = P[Np + o];
-                                            float qi = 0.0f, qj = 0.0f;
-                                            if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, crad[c], crad[o], qi, qj)) {
-                                                P[Np + c] = pi;
-                                                P[Np + o] = pj;
-                                                path[c] = path[c] + qi;
-                                                path[o] = path[o] + qj;
-                                            }
-                                        }
-                                        nh = h + 1;
-                                    }
-                                }
-                                hout[c] = nh;
-                                if (nh < nc_) any = 1;
-                            }
-                        }
-                        if (tid == 0) s_flag[(r + 1) % 3] = 0u;
-                        if (any) s_flag[r % 3] = 1u;
-                        __syncthreads();
-                        uint32_t *t = hin; hin = hout; hout = t;
-                        n_rounds++;
-                        if (!s_flag[r % 3]) break;
-                    }
-                    // ---- check: a pair left out of the list can only have met if a
-                    // circle moved 0.45 * skin or more since the list was built
-                    float vmax = 0.0f;
-                    for (uint32_t j = Np + tid; j < N; j += WG) {
-                        {
-                            uint32_t c = j - Np;
-                            vmax = fmaxf(vmax, moved_bound(save[c], ref[c], path[c]));
-                        }
-                    }
-                    vmax = wg_max<WG>(vmax, s_red);
-                    if (vmax <= 0.45f * skin) break;
-                    // undo the sweep, widen the skin (a stale list alone just
-                    // needs rebuilding), rebuild
-                    for (uint32_t j = Np + tid; j < N; j += WG) P[j] = save[j - Np];
-                    __syncthreads();
-                    n_redos++;
-                    list_valid = false;
-                    if (attempt > 12) { err = 2; break; }
-                    if (vmax < 1e30f) skin = fmaxf(2.0f * skin, 2.5f * vmax);
-                }
-            }
+            if (Nc >= 2 && !err)
+                err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, skin, list_valid,
+                                       n_rounds, n_redos, n_rebuilds, n_pairs);
             // ---- bodies: colliders -> bounds -> Verlet
             const int4 *hdr = staged ? s_col->hdr : ghdr;
             const float4 *geo = staged ? s_col->geo : ggeo;
@@ -408,22 +477,17 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
             if constexpr (MODE == 2) gprev[j] = Q[j];
         }
     }
-    // statistics (lane 0 holds uniform values except pairs)
-    uint32_t pairs_total = n_pairs;
-    for (int o = 32; o > 0; o >>= 1) pairs_total += __shfl_xor(pairs_total, o, 64);
-    __syncthreads();
-    if ((tid & 63) == 0) s_u32[tid >> 6] = pairs_total;
+    // statistics (uniform across the workgroup)
     __syncthreads();
     if (tid == 0) {
-        uint32_t tot = 0;
-        for (int w = 0; w < WG / 64; w++) tot += s_u32[w];
         st->skin = skin;
         st->list_valid = list_valid ? 1u : 0u;
         st->err = err;
         st->rounds += n_rounds;
         st->redos += n_redos;
         st->rebuilds += n_rebuilds;
-        if (n_rebuilds) st->pairs = tot / 2;  // each pair is listed by both circles
+        st->pairs = n_pairs;
+        if (Nc >= 2) st->ws_ready = 1u;
     }
     __syncthreads();
 }
@@ -436,7 +500,7 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
     __shared__ ColliderStage s_col;
     __shared__ float s_red[WG / 64];
     __shared__ uint32_t s_u32[WG / 64];
-    __shared__ uint32_t s_flag[4];
+    __shared__ CCShared s_cc;
     __shared__ uint32_t s_next;
     for (;;) {
         if (threadIdx.x == 0) s_next = atomicAdd(queue, 1u);
@@ -449,11 +513,11 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
         const uint32_t steps = d.steps < launch_steps ? d.steps : launch_steps;
         if (steps == 0 || d.N == 0) continue;
         if (16ull * d.N <= lds_bytes)
-            run_scene<WG, 2>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, s_flag);
+            run_scene<WG, 2>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
         else if (8ull * d.N <= lds_bytes)
-            run_scene<WG, 1>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, s_flag);
+            run_scene<WG, 1>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
         else
-            run_scene<WG, 0>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, s_flag);
+            run_scene<WG, 0>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
     }
 }
 
@@ -609,7 +673,7 @@ int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::s
     img.acc_pending = s.any_initial_acc();
     d.rmax = 0.0f;
     for (float r : s.c_r) d.rmax = std::max(d.rmax, r);
-    img.skin0 = std::max(0.5f * d.rmax, 0.05f);
+    img.skin0 = std::max(0.25f * d.rmax, 0.025f);  // per circle: a pair's margin is the sum of both
     return BD_OK;
 }
 

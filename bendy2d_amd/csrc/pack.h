@@ -25,7 +25,7 @@ struct LinkS { uint16_t a, b; float t; };
 
 constexpr uint32_t kScenePinBit = 0x8000u;
 constexpr uint32_t kSceneMaxPoints = 32767;
-constexpr uint32_t kCandCap = 64;  // candidate list capacity per scene: kCandCap * Nc entries
+constexpr uint32_t kCandCap = 32;  // candidate list capacity per scene: kCandCap * Nc entries
 
 // Scene descriptor: read-only in the kernel.
 struct SceneDesc {
@@ -51,19 +51,19 @@ constexpr uint32_t kFlagLds = 4u;         // positions held in LDS
 
 // Per-scene state that persists between launches (written by the kernel).
 struct SceneState {
-    float skin;          // candidate-list margin of the circle-pair schedule
+    float skin;          // base candidate margin of the circle-pair schedule
     uint32_t list_valid; // candidate list built and still usable
     uint32_t err;        // nonzero: the circle-pair schedule failed
     uint32_t rounds;     // schedule rounds executed (statistics)
-    uint32_t redos;      // sweeps redone with a larger skin
+    uint32_t redos;      // sweeps redone with larger skins
     uint32_t rebuilds;   // candidate lists built
     uint32_t pairs;      // candidate pairs in the last list
-    uint32_t pad;
+    uint32_t ws_ready;   // per-circle skin state initialised
 };
 
 // Byte offsets of the circle-pair workspace inside ws (relative to ws_off).
 struct CCLayout {
-    uint32_t ref, save, path, cnt, off, cand, head, key, cell, start, fill, items, bytes;
+    uint32_t ref, save, path, skin, need, cnt, off, cand, head, stamp, wl, key, cell, start, fill, items, bytes;
 };
 
 BD_HD uint32_t align16(uint64_t v) { return (uint32_t)((v + 15) & ~uint64_t(15)); }
@@ -74,10 +74,14 @@ BD_HD CCLayout cc_layout(uint32_t Nc, uint32_t T) {
     c.ref = (uint32_t)o; o = align16(o + 8ull * Nc);
     c.save = (uint32_t)o; o = align16(o + 8ull * Nc);
     c.path = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.skin = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.need = (uint32_t)o; o = align16(o + 4ull * Nc);
     c.cnt = (uint32_t)o; o = align16(o + 4ull * Nc);
     c.off = (uint32_t)o; o = align16(o + 4ull * (Nc + 1));
     c.cand = (uint32_t)o; o = align16(o + 2ull * kCandCap * Nc);
-    c.head = (uint32_t)o; o = align16(o + 8ull * Nc);
+    c.head = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.stamp = (uint32_t)o; o = align16(o + 4ull * Nc);
+    c.wl = (uint32_t)o; o = align16(o + 4ull * 2 * (Nc / 2 + 1));
     c.key = (uint32_t)o; o = align16(o + 4ull * Nc);
     c.cell = (uint32_t)o; o = align16(o + 8ull * Nc);
     c.start = (uint32_t)o; o = align16(o + 4ull * (T + 1));

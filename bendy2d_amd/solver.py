@@ -341,6 +341,13 @@ class Batch:
         check(lib().bd_batch_last_timing(self._h, ctypes.byref(ms), ctypes.byref(n)), "batch_last_timing")
         return ms.value, n.value
 
+    def scene_status(self, i):
+        f = ctypes.c_float()
+        u = [ctypes.c_uint32() for _ in range(5)]
+        check(lib().bd_batch_scene_status(self._h, int(i), ctypes.byref(f), *[ctypes.byref(x) for x in u]),
+              "batch_scene_status")
+        return dict(zip(("skin", "err", "rounds", "redos", "rebuilds", "pairs"), [f.value] + [x.value for x in u]))
+
     def cc_stats(self):
         v = [ctypes.c_uint64() for _ in range(4)]
         check(lib().bd_batch_cc_stats(self._h, *[ctypes.byref(x) for x in v]), "batch_cc_stats")

@@ -311,6 +311,11 @@ int32_t bd_batch_last_timing(const bd_batch *b, float *ms, uint32_t *launches);
  * whose schedule failed (their results are not valid). */
 int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64_t *rebuilds,
                           uint64_t *err_scenes);
+/* Circle-pair schedule state of one scene: base skin, error kind (0 none,
+ * 1 candidate capacity, 2 no valid sweep in 13 attempts, 3 stalled
+ * schedule), rounds, redos, rebuilds and candidate pairs. */
+int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *err, uint32_t *rounds,
+                              uint32_t *redos, uint32_t *rebuilds, uint32_t *pairs);
 
 /* ------------------------------------------------------------------ */
 /* Serving: continuous batching of scene-advance requests              */
