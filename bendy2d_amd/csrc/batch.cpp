@@ -443,6 +443,19 @@ int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *er
     return BD_OK;
 }
 
+int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *build, double *sweep, double *bodies) {
+    if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
+    SceneState st{};
+    if (b->uploaded && hipMemcpy(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(BD_ERR_DEVICE, "state readback failed");
+    const double tick = 1.0 / 100.0e6;  // s_memrealtime runs at 100 MHz
+    if (links) *links = st.t_links * tick;
+    if (build) *build = st.t_build * tick;
+    if (sweep) *sweep = st.t_sweep * tick;
+    if (bodies) *bodies = st.t_bodies * tick;
+    return BD_OK;
+}
+
 int32_t bd_batch_autotune(bd_batch *b, float dt, uint32_t steps) {
     if (!b) return BD_ERR_ARG;
     std::string err;

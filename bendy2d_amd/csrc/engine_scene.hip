@@ -141,25 +141,31 @@ struct CCWork {
     uint32_t *start, *fill, *items;
 };
 
+constexpr uint32_t kOutCap = 256;  // circles with an outlying skin, per list build
+
 struct CCShared {
+    uint16_t out[kOutCap];   // outlier circles of the current list
+    uint32_t hist[33];       // skins by log2(skin / base)
+    uint32_t nout;
     float red3[3 * 16];      // workgroup sums
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
     uint32_t flag;
 };
 
-// Candidates of circle c: every o != c with |ref_c - ref_o| < r_c + r_o +
-// skin_c + skin_o.  Cells have size h and every circle searches the
-// (2R+1)^2 cells around its own, R covering the largest possible pair
-// distance 2 (r_max + skin_max), so each pair is found from both ends.
-// Writes the candidates to out when given; returns the count.
-__device__ __forceinline__ uint32_t gather_candidates(uint32_t c, const CCWork &w, const float *crad, int R,
-                                                      uint32_t tmask, uint16_t *out) {
+// Candidate pairs are the pairs closer than r_c + r_o + skin_c + skin_o when
+// the list is built.  A normal circle (skin <= s_cell) finds its normal
+// partners in the 3x3 cells of size h = 2 (r_max + s_cell) around it.  An
+// outlier (larger skin) searches the cells its reach covers (every circle if
+// that is far) and finds all its partners, writing itself into its normal
+// partners' lists too, so normal circles never search far.
+__device__ __forceinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, const float *crad, float s_cell,
+                                                  uint32_t tmask, uint16_t *out) {
     const float2 p = w.ref[c];
     const float rc = crad[c], sc = w.skin[c];
     const int2 ce = w.cell[c];
     uint32_t n = 0;
-    for (int dy = -R; dy <= R; dy++)
-        for (int dx = -R; dx <= R; dx++) {
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
             int cx = ce.x + dx, cy = ce.y + dy;
             uint32_t kk = cell_hash(cx, cy) & tmask;
             for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
@@ -167,9 +173,11 @@ __device__ __forceinline__ uint32_t gather_candidates(uint32_t c, const CCWork &
                 if (o == c) continue;
                 int2 co = w.cell[o];
                 if (co.x != cx || co.y != cy) continue;
+                float so = w.skin[o];
+                if (so > s_cell) continue;  // an outlier lists this pair itself
                 float2 po = w.ref[o];
                 float ex = p.x - po.x, ey = p.y - po.y;
-                float lim = (rc + crad[o]) + (sc + w.skin[o]);
+                float lim = (rc + crad[o]) + (sc + so);
                 if (ex * ex + ey * ey < lim * lim) {
                     if (out) out[n] = (uint16_t)o;
                     n++;
@@ -177,6 +185,52 @@ __device__ __forceinline__ uint32_t gather_candidates(uint32_t c, const CCWork &
             }
         }
     return n;
+}
+
+// Outlier c: all partners, found by the whole workgroup (the candidate
+// circles are split across threads).  MODE 0 counts (into cnt[c], and 1 into
+// each normal partner's count); MODE 1 appends each partner to c's list and c
+// to each normal partner's list, at the lists' cursors.
+template <int WG, int MODE>
+__device__ __forceinline__ void gather_outlier(uint32_t c, uint32_t Nc, const CCWork &w, const float *crad,
+                                               float rmax, float s_cell, float smax, double h, uint32_t tmask,
+                                               uint32_t *cursor) {
+    const float2 p = w.ref[c];
+    const float rc = crad[c], sc = w.skin[c];
+    auto test = [&](uint32_t o) {
+        float2 po = w.ref[o];
+        float ex = p.x - po.x, ey = p.y - po.y;
+        float so = w.skin[o];
+        float lim = (rc + crad[o]) + (sc + so);
+        if (ex * ex + ey * ey < lim * lim) {
+            if (MODE == 0) atomicAdd(&w.cnt[c], 1u);
+            else w.cand[atomicAdd(&cursor[c], 1u)] = (uint16_t)o;
+            if (so <= s_cell) {
+                if (MODE == 0) atomicAdd(&w.cnt[o], 1u);
+                else w.cand[atomicAdd(&cursor[o], 1u)] = (uint16_t)c;
+            }
+        }
+    };
+    const double reach = ((double)rc + (double)rmax + (double)sc + (double)smax) / h;
+    if (!(reach <= 8.0)) {
+        for (uint32_t o = threadIdx.x; o < Nc; o += WG)
+            if (o != c) test(o);
+        return;
+    }
+    const int R = (int)ceil(reach);
+    const int side = 2 * R + 1;
+    const int2 ce = w.cell[c];
+    for (int q = threadIdx.x; q < side * side; q += WG) {
+        int cx = ce.x + (q % side) - R, cy = ce.y + (q / side) - R;
+        uint32_t kk = cell_hash(cx, cy) & tmask;
+        for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
+            uint32_t o = w.items[s];
+            if (o == c) continue;
+            int2 co = w.cell[o];
+            if (co.x != cx || co.y != cy) continue;
+            test(o);
+        }
+    }
 }
 
 // One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
@@ -195,7 +249,7 @@ template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
                                  const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, float &sbase,
                                  bool &list_valid, uint32_t &n_rounds, uint32_t &n_redos, uint32_t &n_rebuilds,
-                                 uint32_t &n_pairs) {
+                                 uint32_t &n_pairs, uint64_t &t_build, uint32_t &since_build) {
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
     for (int attempt = 0;; attempt++) {
@@ -219,18 +273,37 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         }
         if (rebuild) {
             n_rebuilds++;
-            // per-circle skins: the base, or what the circle's last sweep pushed it
+            const uint64_t tb0 = wall_clock64();
+            // per-circle skins: the base, or what the circle's recent sweeps pushed it;
+            // s_cell: the skin at most kOutCap circles exceed (log2 histogram)
+            if (tid <= 32) sh->hist[tid] = 0u;
+            __syncthreads();
             float smax = 0.0f;
             for (uint32_t c = tid; c < Nc; c += WG) {
                 float sk = fmaxf(sbase, w.need[c]);
                 w.skin[c] = sk;
                 smax = fmaxf(smax, sk);
+                int b = 0;
+                if (sk > sbase) b = sk <= 3.0e38f ? min(32, 1 + (int)floorf(log2f(sk / sbase))) : 32;
+                atomicAdd(&sh->hist[b], 1u);
             }
             smax = wg_max<WG>(smax, s_red);
-            // cell size: the base pair distance, grown so the reach stays <= 4 cells
-            const double reach = 2.0 * ((double)rmax + (double)smax);
-            const double h = fmax(2.0 * ((double)rmax + (double)sbase), reach / 4.0) * 1.001 + 1e-30;
-            const int R = (int)ceil(reach / h);
+            int B = 0;
+            {
+                uint32_t above = 0;
+                for (int b = 32; b >= 1; b--) {
+                    if (above + sh->hist[b] > kOutCap) { B = b; break; }
+                    above += sh->hist[b];
+                }
+            }
+            const float s_cell = fminf(smax, sbase * exp2f((float)B));
+            if (tid == 0) sh->nout = 0u;
+            __syncthreads();
+            for (uint32_t c = tid; c < Nc; c += WG)
+                if (w.skin[c] > s_cell) sh->out[atomicAdd(&sh->nout, 1u)] = (uint16_t)c;
+            __syncthreads();
+            const uint32_t n_out = sh->nout;
+            const double h = 2.0 * ((double)rmax + (double)s_cell) * 1.001 + 1e-30;
             const double inv_h = 1.0 / h;
             for (uint32_t b = tid; b <= T; b += WG) w.start[b] = 0u;
             for (uint32_t b = tid; b < T; b += WG) w.fill[b] = 0u;
@@ -251,11 +324,18 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 w.items[w.start[kk] + atomicAdd(&w.fill[kk], 1u)] = c;
             }
             __syncthreads();
+            // counts: normal circles their normal partners (kept in stamp until the
+            // sweep needs it), then outliers add themselves to their partners
             for (uint32_t c = tid; c < Nc; c += WG) {
-                uint32_t n = gather_candidates(c, w, crad, R, tmask, nullptr);
+                uint32_t n = w.skin[c] > s_cell ? 0u : gather_normal(c, w, crad, s_cell, tmask, nullptr);
                 w.cnt[c] = n;
-                w.off[c] = n;
+                w.stamp[c] = n;
             }
+            __syncthreads();
+            for (uint32_t k = 0; k < n_out; k++)
+                gather_outlier<WG, 0>(sh->out[k], Nc, w, crad, rmax, s_cell, smax, h, tmask, nullptr);
+            __syncthreads();
+            for (uint32_t c = tid; c < Nc; c += WG) w.off[c] = w.cnt[c];
             if (tid == 0) w.off[Nc] = 0u;
             __syncthreads();
             wg_scan_global<WG>(w.off, Nc + 1, s_u32);
@@ -270,9 +350,19 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 if (sbase < 1e-4f * fmaxf(rmax, 1e-3f)) return 1u;
                 continue;
             }
+            // lists: normal partners first, outliers append at a cursor (in head)
+            for (uint32_t c = tid; c < Nc; c += WG) {
+                uint32_t own = 0;
+                if (w.skin[c] <= s_cell) own = gather_normal(c, w, crad, s_cell, tmask, w.cand + w.off[c]);
+                w.head[c] = w.off[c] + own;
+            }
+            __syncthreads();
+            for (uint32_t k = 0; k < n_out; k++)
+                gather_outlier<WG, 1>(sh->out[k], Nc, w, crad, rmax, s_cell, smax, h, tmask, w.head);
+            __syncthreads();
             for (uint32_t c = tid; c < Nc; c += WG) {
                 uint16_t *my = w.cand + w.off[c];
-                uint32_t n = gather_candidates(c, w, crad, R, tmask, my);
+                const uint32_t n = w.cnt[c];
                 for (uint32_t x = 1; x < n; x++) {  // ascending: the reference's pair order
                     uint16_t v = my[x];
                     uint32_t y = x;
@@ -280,9 +370,15 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                     my[y] = v;
                 }
             }
+            // base skin: rebuilds every few sweeps cost more than larger lists
+            if (since_build < 4) sbase = fminf(1.25f * sbase, 0.5f * fmaxf(rmax, 1e-3f));
+            else if (since_build > 32) sbase = fmaxf(0.8f * sbase, 0.05f * fmaxf(rmax, 1e-3f));
+            since_build = 0;
             list_valid = true;
             n_pairs = total / 2;
             u = make_float2(0.0f, 0.0f);
+            __syncthreads();
+            t_build += wall_clock64() - tb0;
         }
         // ---- sweep: worklist rounds
         for (uint32_t c = tid; c < Nc; c += WG) {
@@ -346,11 +442,15 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             if (w.head[c] != w.cnt[c]) left = 1.0f;
             float pth = w.path[c];
             float v = moved_bound(w.save[c], w.ref[c], u, pth);
-            w.need[c] = (pth <= 3.0e38f) ? 2.5f * pth : 0.0f;  // the sweep's own pushes
+            // the sweep's own pushes, decaying slowly so circles in lasting contact keep their skin
+            w.need[c] = (pth <= 3.0e38f) ? fmaxf(2.5f * pth, 0.75f * w.need[c]) : 0.0f;
             if (!(v <= 0.8f * w.skin[c])) fail = 1.0f;
         }
         if (wg_max<WG>(left, s_red) > 0.0f) return 3u;
-        if (wg_max<WG>(fail, s_red) == 0.0f) return 0u;
+        if (wg_max<WG>(fail, s_red) == 0.0f) {
+            since_build++;
+            return 0u;
+        }
         // undo and rebuild with the skins the circles needed
         for (uint32_t c = tid; c < Nc; c += WG) P[Np + c] = w.save[c];
         __syncthreads();
@@ -413,12 +513,15 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
     bool list_valid = st->list_valid != 0;
     uint32_t err = st->err;
     uint32_t n_rounds = 0, n_redos = 0, n_rebuilds = 0, n_pairs = st->pairs;
+    uint64_t t_links = 0, t_build = 0, t_sweep = 0, t_bodies = 0;
+    uint32_t since_build = st->since_build;
     bool acc = (d.flags & kFlagAcc) != 0;
     const float2 *acc0 = (const float2 *)(base + d.acc0_off);
 
     for (uint32_t step = 0; step < steps; step++) {
         for (uint32_t sub = 0; sub < d.sub_steps; sub++) {
             // ---- links: dependency layers, k sweeps
+            const uint64_t tl0 = wall_clock64();
             for (uint32_t it = 0; it < d.link_iters; it++) {
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = loff[L], hi = loff[L + 1];
@@ -438,11 +541,17 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
                     __syncthreads();
                 }
             }
+            t_links += wall_clock64() - tl0;
             // ---- circle pairs
             if (Nc >= 2 && !err)
+            {
+                const uint64_t t0 = wall_clock64();
                 err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, skin, list_valid,
-                                       n_rounds, n_redos, n_rebuilds, n_pairs);
+                                       n_rounds, n_redos, n_rebuilds, n_pairs, t_build, since_build);
+                t_sweep += wall_clock64() - t0;
+            }
             // ---- bodies: colliders -> bounds -> Verlet
+            const uint64_t tq0 = wall_clock64();
             const int4 *hdr = staged ? s_col->hdr : ghdr;
             const float4 *geo = staged ? s_col->geo : ggeo;
             const float4 *pl = staged ? s_col->pl : gpl;
@@ -468,6 +577,7 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
             }
             acc = false;
             __syncthreads();
+            t_bodies += wall_clock64() - tq0;
         }
     }
     // write back
@@ -488,6 +598,11 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
         st->rebuilds += n_rebuilds;
         st->pairs = n_pairs;
         if (Nc >= 2) st->ws_ready = 1u;
+        st->t_links += t_links;
+        st->t_build += t_build;
+        st->t_sweep += t_sweep - t_build;
+        st->t_bodies += t_bodies;
+        st->since_build = since_build;
     }
     __syncthreads();
 }

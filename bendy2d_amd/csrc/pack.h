@@ -59,6 +59,9 @@ struct SceneState {
     uint32_t rebuilds;   // candidate lists built
     uint32_t pairs;      // candidate pairs in the last list
     uint32_t ws_ready;   // per-circle skin state initialised
+    uint64_t t_links, t_build, t_sweep, t_bodies;  // phase times, 100 MHz ticks (statistics)
+    uint32_t since_build;  // sweeps since the candidate list was built
+    uint32_t pad2;
 };
 
 // Byte offsets of the circle-pair workspace inside ws (relative to ws_off).

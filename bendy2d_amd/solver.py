@@ -348,6 +348,11 @@ class Batch:
               "batch_scene_status")
         return dict(zip(("skin", "err", "rounds", "redos", "rebuilds", "pairs"), [f.value] + [x.value for x in u]))
 
+    def scene_times(self, i):
+        v = [ctypes.c_double() for _ in range(4)]
+        check(lib().bd_batch_scene_times(self._h, int(i), *[ctypes.byref(x) for x in v]), "batch_scene_times")
+        return dict(zip(("links", "build", "sweep", "bodies"), [x.value for x in v]))
+
     def cc_stats(self):
         v = [ctypes.c_uint64() for _ in range(4)]
         check(lib().bd_batch_cc_stats(self._h, *[ctypes.byref(x) for x in v]), "batch_cc_stats")

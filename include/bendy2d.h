@@ -316,6 +316,9 @@ int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64
  * schedule), rounds, redos, rebuilds and candidate pairs. */
 int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *err, uint32_t *rounds,
                               uint32_t *redos, uint32_t *rebuilds, uint32_t *pairs);
+/* Device time a scene spent per phase so far, in seconds: link layers,
+ * candidate-list builds, pair sweeps (rounds and checks), bodies. */
+int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *build, double *sweep, double *bodies);
 
 /* ------------------------------------------------------------------ */
 /* Serving: continuous batching of scene-advance requests              */
