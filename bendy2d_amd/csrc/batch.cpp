@@ -443,16 +443,45 @@ int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *er
     return BD_OK;
 }
 
+// Period of the device wall clock that the kernels' phase timers read.
+static double wall_tick_seconds() {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        khz = 100000;
+    return 1.0 / (1000.0 * (double)khz);
+}
+
 int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *build, double *sweep, double *bodies) {
     if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
     SceneState st{};
     if (b->uploaded && hipMemcpy(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
         return fail(BD_ERR_DEVICE, "state readback failed");
-    const double tick = 1.0 / 100.0e6;  // s_memrealtime runs at 100 MHz
+    const double tick = wall_tick_seconds();
     if (links) *links = st.t_links * tick;
     if (build) *build = st.t_build * tick;
     if (sweep) *sweep = st.t_sweep * tick;
     if (bodies) *bodies = st.t_bodies * tick;
+    return BD_OK;
+}
+
+int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t *n_far, uint32_t *max_list,
+                             float *s_cell, float *smax, double *t_count, double *t_lists, double *t_sort,
+                             uint32_t *kept) {
+    if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
+    SceneState st{};
+    if (b->uploaded && hipMemcpy(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(BD_ERR_DEVICE, "state readback failed");
+    const double tick = wall_tick_seconds();
+    if (n_out) *n_out = st.n_out;
+    if (n_far) *n_far = st.n_far;
+    if (max_list) *max_list = st.max_list;
+    if (s_cell) *s_cell = st.s_cell;
+    if (smax) *smax = st.smax;
+    if (t_count) *t_count = st.t_count * tick;
+    if (t_lists) *t_lists = st.t_lists * tick;
+    if (t_sort) *t_sort = st.t_sort * tick;
+    if (kept) *kept = st.kept;
     return BD_OK;
 }
 

@@ -44,6 +44,11 @@ struct ColliderStage {
     float4 pl[kStagePlanes];
 };
 
+// A value every lane of the workgroup holds alike, moved to a scalar register
+// (the compiler cannot see that values read back from LDS are uniform).
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
+
 template <int WG>
 __device__ __forceinline__ float wg_max(float v, float *s_red) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -52,8 +57,9 @@ __device__ __forceinline__ float wg_max(float v, float *s_red) {
     if ((threadIdx.x & 63) == 0) s_red[wave] = v;
     __syncthreads();
     float m = s_red[0];
+#pragma unroll 2
     for (int w = 1; w < WG / 64; w++) m = fmaxf(m, s_red[w]);
-    return m;
+    return uni(m);
 }
 
 // Exclusive scan in place over a[0..n) in global memory by one workgroup.
@@ -89,7 +95,8 @@ __device__ void wg_scan_global(uint32_t *a, uint32_t n, uint32_t *s_wave) {
 }
 
 // Upper bound on how far a circle can have been from its list-build position
-// `r0` during a sweep that started at `p` and pushed it `path` in total.  A
+// `r0` during a sweep that started at `p` and took it at most `path` away from
+// `p` (its largest excursion, see below).  A
 // circle at a non-finite position takes part in no pair (every comparison
 // with it is false), so it cannot invalidate the list; one whose list-build
 // position was non-finite but is finite now forces a rebuild.
@@ -105,6 +112,17 @@ __device__ __forceinline__ float moved_bound(float2 p, float2 r0, float2 u, floa
     return v == v ? v : INFINITY;
 }
 
+// Largest distance from the sweep's start position `s` reached so far, after
+// a push moved the circle to `p`.  The distance is rounded up by a relative
+// 2^-20, far above float rounding, so the bound never falls below the truth.
+// A circle pushed back and forth keeps a small excursion where the sum of its
+// pushes would grow without end.
+__device__ __forceinline__ float excursion(float cur, float2 p, float2 s) {
+    float dx = p.x - s.x, dy = p.y - s.y;
+    float d = sqrtf(dx * dx + dy * dy) * (1.0f + 0x1p-20f);
+    return fmaxf(cur, d);
+}
+
 // Sum of (x, y, count) over the workgroup (every thread gets the result).
 template <int WG>
 __device__ __forceinline__ float3 wg_sum3(float3 v, float *s_red3) {
@@ -118,8 +136,9 @@ __device__ __forceinline__ float3 wg_sum3(float3 v, float *s_red3) {
     if ((threadIdx.x & 63) == 0) { s_red3[3 * wave] = v.x; s_red3[3 * wave + 1] = v.y; s_red3[3 * wave + 2] = v.z; }
     __syncthreads();
     float3 t = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll 2
     for (int w = 0; w < WG / 64; w++) { t.x += s_red3[3 * w]; t.y += s_red3[3 * w + 1]; t.z += s_red3[3 * w + 2]; }
-    return t;
+    return make_float3(uni(t.x), uni(t.y), uni(t.z));
 }
 
 __device__ __forceinline__ int scell(float x, double inv_h) {
@@ -136,102 +155,295 @@ struct CCWork {
     float *path, *skin, *need;
     uint32_t *cnt, *off;
     uint16_t *cand;
+    float2 *traj;  // per list entry: the circle's position after that pair ran
     uint32_t *head, *stamp, *wl, *key;
-    int2 *cell;
+    float4 *rec;  // circles in bucket order: x, y, radius + skin, index (| kOutlierBit)
     uint32_t *start, *fill, *items;
 };
 
-constexpr uint32_t kOutCap = 256;  // circles with an outlying skin, per list build
+constexpr uint32_t kOutlierBit = 0x80000000u;
+
+constexpr uint32_t kOutCap = 256;   // circles with an outlying skin, per list build
+constexpr uint32_t kMidList = 256;  // longer candidate lists are sorted by the whole workgroup
 
 struct CCShared {
     uint16_t out[kOutCap];   // outlier circles of the current list
     uint32_t hist[33];       // skins by log2(skin / base)
     uint32_t nout;
+    uint32_t nmid, nlong;    // candidate lists sorted by a wavefront / by the workgroup
     float red3[3 * 16];      // workgroup sums
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
-    uint32_t flag;
 };
 
-// Candidate pairs are the pairs closer than r_c + r_o + skin_c + skin_o when
-// the list is built.  A normal circle (skin <= s_cell) finds its normal
-// partners in the 3x3 cells of size h = 2 (r_max + s_cell) around it.  An
-// outlier (larger skin) searches the cells its reach covers (every circle if
-// that is far) and finds all its partners, writing itself into its normal
-// partners' lists too, so normal circles never search far.
-__device__ __forceinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, const float *crad, float s_cell,
+// Candidate pairs are the pairs closer than (r_c + skin_c) + (r_o + skin_o)
+// when the list is built.  A normal circle (skin <= s_cell) finds its normal
+// partners in the 3x3 cells of size h = 2 (r_max + s_cell) around it: the
+// circles sit in hash buckets as 16-byte records, and each distinct bucket of
+// the 9 is scanned once (a record from another cell that hashes there is
+// still a true partner when it passes the distance test, and it is in no
+// other scanned bucket).  An outlier (larger skin) finds all its partners
+// itself and writes itself into its normal partners' lists too, so normal
+// circles never search far.  out == nullptr counts only.
+__device__ __forceinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, float2 p, float ec, double inv_h,
                                                   uint32_t tmask, uint16_t *out) {
-    const float2 p = w.ref[c];
-    const float rc = crad[c], sc = w.skin[c];
-    const int2 ce = w.cell[c];
+    const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
+    uint32_t kk[9];
+    bool dup[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) kk[i] = cell_hash(cx0 + (i % 3) - 1, cy0 + (i / 3) - 1) & tmask;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        dup[i] = false;
+#pragma unroll
+        for (int j = 0; j < i; j++) dup[i] = dup[i] || kk[j] == kk[i];
+    }
     uint32_t n = 0;
-    for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-            int cx = ce.x + dx, cy = ce.y + dy;
-            uint32_t kk = cell_hash(cx, cy) & tmask;
-            for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
-                uint32_t o = w.items[s];
-                if (o == c) continue;
-                int2 co = w.cell[o];
-                if (co.x != cx || co.y != cy) continue;
-                float so = w.skin[o];
-                if (so > s_cell) continue;  // an outlier lists this pair itself
-                float2 po = w.ref[o];
-                float ex = p.x - po.x, ey = p.y - po.y;
-                float lim = (rc + crad[o]) + (sc + so);
-                if (ex * ex + ey * ey < lim * lim) {
-                    if (out) out[n] = (uint16_t)o;
-                    n++;
-                }
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        if (dup[i]) continue;
+        const uint32_t s0 = w.start[kk[i]], s1 = w.start[kk[i] + 1];
+#pragma unroll 4
+        for (uint32_t s = s0; s < s1; s++) {
+            const float4 r = w.rec[s];
+            const uint32_t o = __float_as_uint(r.w);
+            if (o == c || (o & kOutlierBit)) continue;  // an outlier lists its pairs itself
+            float ex = p.x - r.x, ey = p.y - r.y;
+            float lim = ec + r.z;
+            if (ex * ex + ey * ey < lim * lim) {
+                if (out) out[n] = (uint16_t)o;
+                n++;
             }
         }
+    }
     return n;
 }
 
-// Outlier c: all partners, found by the whole workgroup (the candidate
-// circles are split across threads).  MODE 0 counts (into cnt[c], and 1 into
-// each normal partner's count); MODE 1 appends each partner to c's list and c
-// to each normal partner's list, at the lists' cursors.
+// Outliers' partners.  A near outlier (its normal partners lie within 8
+// cells) is searched by one wavefront over those cells, a far one by the whole
+// workgroup over every circle, and pairs of two outliers by all pairs among
+// the outliers.  An outlier-normal pair is found once, by the outlier, and
+// goes into both lists; an outlier-outlier pair is found once per direction
+// (the test is symmetric) and goes into the first one's list.  MODE 0 counts
+// into cnt, MODE 1 appends at the lists' cursors.  Returns the far outliers.
 template <int WG, int MODE>
-__device__ __forceinline__ void gather_outlier(uint32_t c, uint32_t Nc, const CCWork &w, const float *crad,
-                                               float rmax, float s_cell, float smax, double h, uint32_t tmask,
-                                               uint32_t *cursor) {
-    const float2 p = w.ref[c];
-    const float rc = crad[c], sc = w.skin[c];
-    auto test = [&](uint32_t o) {
-        float2 po = w.ref[o];
-        float ex = p.x - po.x, ey = p.y - po.y;
-        float so = w.skin[o];
-        float lim = (rc + crad[o]) + (sc + so);
-        if (ex * ex + ey * ey < lim * lim) {
-            if (MODE == 0) atomicAdd(&w.cnt[c], 1u);
-            else w.cand[atomicAdd(&cursor[c], 1u)] = (uint16_t)o;
-            if (so <= s_cell) {
-                if (MODE == 0) atomicAdd(&w.cnt[o], 1u);
-                else w.cand[atomicAdd(&cursor[o], 1u)] = (uint16_t)c;
+__device__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, const float *crad, const uint16_t *outs,
+                                    uint32_t n_out, float rmax, float s_cell, double h, double inv_h,
+                                    uint32_t tmask, uint32_t *cursor) {
+    auto add = [&](uint32_t c, uint32_t o) {
+        if (MODE == 0) atomicAdd(&w.cnt[c], 1u);
+        else w.cand[atomicAdd(&cursor[c], 1u)] = (uint16_t)o;
+    };
+    auto reach_of = [&](float ec) { return ((double)ec + (double)rmax + (double)s_cell) / h; };
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t k = wave; k < n_out; k += WG / 64) {
+        const uint32_t c = outs[k];
+        const float2 p = w.ref[c];
+        const float ec = crad[c] + w.skin[c];
+        const double reach = reach_of(ec);
+        if (!(reach <= 8.0)) continue;
+        const int R = (int)ceil(reach), side = 2 * R + 1;
+        const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
+        for (int q = (int)lane; q < side * side; q += 64) {
+            const int cx = cx0 + (q % side) - R, cy = cy0 + (q / side) - R;
+            const uint32_t kk = cell_hash(cx, cy) & tmask;
+            for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
+                const float4 r = w.rec[s];
+                const uint32_t o = __float_as_uint(r.w);
+                if (o & kOutlierBit) continue;  // outliers: below
+                if (scell(r.x, inv_h) != cx || scell(r.y, inv_h) != cy) continue;  // another cell, same bucket
+                float ex = p.x - r.x, ey = p.y - r.y;
+                float lim = ec + r.z;
+                if (ex * ex + ey * ey < lim * lim) { add(c, o); add(o, c); }
             }
         }
-    };
-    const double reach = ((double)rc + (double)rmax + (double)sc + (double)smax) / h;
-    if (!(reach <= 8.0)) {
-        for (uint32_t o = threadIdx.x; o < Nc; o += WG)
-            if (o != c) test(o);
-        return;
     }
-    const int R = (int)ceil(reach);
-    const int side = 2 * R + 1;
-    const int2 ce = w.cell[c];
-    for (int q = threadIdx.x; q < side * side; q += WG) {
-        int cx = ce.x + (q % side) - R, cy = ce.y + (q / side) - R;
-        uint32_t kk = cell_hash(cx, cy) & tmask;
-        for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
-            uint32_t o = w.items[s];
-            if (o == c) continue;
-            int2 co = w.cell[o];
-            if (co.x != cx || co.y != cy) continue;
-            test(o);
+    uint32_t n_far = 0;
+    for (uint32_t k = 0; k < n_out; k++) {
+        const uint32_t c = outs[k];
+        const float ec = crad[c] + w.skin[c];
+        if (reach_of(ec) <= 8.0) continue;
+        n_far++;
+        const float2 p = w.ref[c];
+        for (uint32_t s = threadIdx.x; s < Nc; s += WG) {
+            const float4 r = w.rec[s];
+            const uint32_t o = __float_as_uint(r.w);
+            if (o & kOutlierBit) continue;
+            float ex = p.x - r.x, ey = p.y - r.y;
+            float lim = ec + r.z;
+            if (ex * ex + ey * ey < lim * lim) { add(c, o); add(o, c); }
         }
     }
+    for (uint32_t q = threadIdx.x; q < n_out * n_out; q += WG) {
+        const uint32_t a = q / n_out, b = q - a * n_out;
+        if (a == b) continue;
+        const uint32_t c = outs[a], o = outs[b];
+        float2 p = w.ref[c], po = w.ref[o];
+        float ex = p.x - po.x, ey = p.y - po.y;
+        float lim = (crad[c] + w.skin[c]) + (crad[o] + w.skin[o]);
+        if (ex * ex + ey * ey < lim * lim) add(c, o);
+    }
+    return n_far;
 }
+
+// Ascending sort of a list of at most K < 65536 entries by one thread: a
+// bitonic network over registers (K a power of two), padding sorting last.
+template <int K>
+__device__ __forceinline__ void sort_in_registers(uint16_t *my, uint32_t n) {
+    uint32_t v[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) v[i] = i < (int)n ? (uint32_t)my[i] : 0xffffffffu;
+#pragma unroll
+    for (int k = 2; k <= K; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < K; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t a = v[i], b = v[l];
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? min(a, b) : max(a, b);
+                    v[l] = up ? max(a, b) : min(a, b);
+                }
+            }
+#pragma unroll
+    for (int i = 0; i < K; i++)
+        if (i < (int)n) my[i] = (uint16_t)v[i];
+}
+
+// Ascending sort of a list of at most kMidList distinct entries by one
+// wavefront: each lane ranks its entries against the whole list, then writes
+// them to their ranks.  Every lane runs the rank loop to the end (the trip
+// count is uniform) before any lane stores, so no entry is overwritten while
+// it is still being read.
+__device__ __forceinline__ void sort_by_wave(uint16_t *my, uint32_t n) {
+    constexpr int M = kMidList / 64;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t v[M], r[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const uint32_t k = lane + 64u * m;
+        v[m] = k < n ? (uint32_t)my[k] : 0xffffffffu;
+        r[m] = 0u;
+    }
+#pragma unroll 8
+    for (uint32_t j = 0; j < n; j++) {
+        const uint32_t x = my[j];
+#pragma unroll
+        for (int m = 0; m < M; m++) r[m] += x < v[m] ? 1u : 0u;
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++)
+        if (lane + 64u * m < n) my[r[m]] = (uint16_t)v[m];
+}
+
+// Cell size of the candidate grid: twice the largest radius plus the skin of
+// normal circles, so a normal pair lies in neighbouring cells.
+__device__ __forceinline__ double cell_size(float rmax, float s_cell) {
+    return 2.0 * ((double)rmax + (double)s_cell) * 1.001 + 1e-30;
+}
+
+// Position of x in c's sorted candidate list, or where it would go; found
+// tells whether the list holds it.
+__device__ __forceinline__ uint32_t list_rank(const CCWork &w, uint32_t c, uint32_t x, bool &found) {
+    const uint16_t *my = w.cand + w.off[c];
+    uint32_t lo = 0, hi = w.cnt[c];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t v = my[mid];
+        if (v == x) { found = true; return mid; }
+        if (v < x) lo = mid + 1; else hi = mid;
+    }
+    found = false;
+    return lo;
+}
+
+// Whether the pair (c, x), missing from the candidate lists, would have
+// overlapped when the reference tests it.  Its turn comes after the pairs of
+// c with smaller partners and those of x with smaller partners (the
+// reference's (i, j) order), so each circle is where the sweep left it after
+// that prefix of its own list: exactly, as long as no other missing pair
+// touched first, which this same test covers for every missing pair.  The
+// overlap test is Circle::solve_circle's (src/circle.rs:32-35) on those
+// positions, so a pair it clears was a no-op in the reference too.
+__device__ __forceinline__ bool missed_pair_touches(const CCWork &w, const float *crad, uint32_t c, uint32_t x) {
+    bool found = false;
+    const uint32_t kc = list_rank(w, c, x, found);
+    if (found) return false;  // on the list: run in order by the sweep
+    const uint32_t kx = list_rank(w, x, c, found);
+    const float2 pc = kc ? w.traj[w.off[c] + kc - 1] : w.save[c];
+    const float2 px = kx ? w.traj[w.off[x] + kx - 1] : w.save[x];
+    const bool c_first = c < x;
+    const float2 pi = c_first ? pc : px, pj = c_first ? px : pc;
+    const float ri = crad[c_first ? c : x], rj = crad[c_first ? x : c];
+    float dx = pi.x - pj.x, dy = pi.y - pj.y;
+    float q = dx * dx + dy * dy;
+    float rs = ri + rj;
+    return q < rs * rs;
+}
+
+// A sweep in which some circles moved further than 0.8 of their skins is
+// still exact when no pair missing from the list touched at its turn.  Pairs
+// of two circles within their skins stayed apart by construction; this looks
+// at every pair with a circle beyond its skin (`over`, with -v in vb; the
+// others hold v <= vmax, v being a circle's bound on its distance from its
+// list-build position, relative to the scene shift, over the whole sweep).
+// A pair whose list-build positions are more than r_c + r_x + v_c + v_x apart
+// stayed apart; the rest get the exact test above.  Returns whether the sweep
+// stands.
+template <int WG>
+__device__ bool overruns_safe(const CCWork &w, const float *crad, const float *vb, const uint16_t *over,
+                              uint32_t n_over, float vmax, float rmax, double h, double inv_h, uint32_t tmask,
+                              float *s_red) {
+    // rounding margin: a fifth of the overrun circle's skin (the margin the
+    // skin test leaves too) plus about 80 ulps of the coordinates
+    auto margin = [&](float2 p, float sk) { return 0.2f * sk + 1e-5f * (fabsf(p.x) + fabsf(p.y) + 1.0f); };
+    auto apart = [&](float2 p, float2 q, float lim) {
+        float dx = p.x - q.x, dy = p.y - q.y;
+        return !(sqrtf(dx * dx + dy * dy) <= lim);  // NaN: a non-finite circle touches nothing
+    };
+    float bad = 0.0f;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t k = wave; k < n_over; k += WG / 64) {
+        const uint32_t c = over[k];
+        const float2 p = w.ref[c];
+        const float rc = crad[c], vc = -vb[c], mc = margin(p, w.skin[c]);
+        const double reach = ((double)rc + (double)vc + (double)mc + (double)rmax + (double)vmax) / h;
+        if (!(reach <= 8.0)) { bad = 1.0f; continue; }
+        const int R = (int)ceil(reach), side = 2 * R + 1;
+        const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
+        for (int q = (int)lane; q < side * side; q += 64) {
+            const uint32_t kk = cell_hash(cx0 + (q % side) - R, cy0 + (q / side) - R) & tmask;
+            for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
+                const float4 r = w.rec[s];
+                const uint32_t x = __float_as_uint(r.w) & ~kOutlierBit;
+                const float vx = vb[x];
+                if (x == c || !(vx >= 0.0f)) continue;  // itself, or beyond its skin: below
+                if (apart(p, make_float2(r.x, r.y), ((rc + crad[x]) + (vc + vx)) + mc)) continue;
+                if (missed_pair_touches(w, crad, c, x)) bad = 1.0f;
+            }
+        }
+    }
+    for (uint32_t q = threadIdx.x; q < n_over * n_over; q += WG) {
+        const uint32_t a = q / n_over, b = q - a * n_over;
+        if (a >= b) continue;
+        const uint32_t c = over[a], o = over[b];
+        const float2 p = w.ref[c];
+        const float lim = ((crad[c] + crad[o]) + (-vb[c] - vb[o])) + margin(p, w.skin[c] + w.skin[o]);
+        if (apart(p, w.ref[o], lim)) continue;
+        if (missed_pair_touches(w, crad, c, o)) bad = 1.0f;
+    }
+    return wg_max<WG>(bad, s_red) == 0.0f;
+}
+
+// Per-scene counters of the circle-pair schedule, kept in registers during a
+// launch (uniform across the workgroup).
+struct CCStats {
+    uint32_t rounds, redos, rebuilds, pairs, since_build, kept;
+    uint32_t n_out, n_far, max_list;
+    float s_cell, smax;
+    uint64_t t_build, t_count, t_lists, t_sort;
+};
 
 // One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
 // candidate pairs run in rounds; pair (c, o), c < o, is ready when o is the
@@ -248,8 +460,7 @@ __device__ __forceinline__ void gather_outlier(uint32_t c, uint32_t Nc, const CC
 template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
                                  const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, float &sbase,
-                                 bool &list_valid, uint32_t &n_rounds, uint32_t &n_redos, uint32_t &n_rebuilds,
-                                 uint32_t &n_pairs, uint64_t &t_build, uint32_t &since_build) {
+                                 bool &list_valid, CCStats &cs) {
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
     for (int attempt = 0;; attempt++) {
@@ -272,7 +483,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             rebuild = wg_max<WG>(bad, s_red) > 0.0f;
         }
         if (rebuild) {
-            n_rebuilds++;
+            cs.rebuilds++;
             const uint64_t tb0 = wall_clock64();
             // per-circle skins: the base, or what the circle's recent sweeps pushed it;
             // s_cell: the skin at most kOutCap circles exceed (log2 histogram)
@@ -296,14 +507,14 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                     above += sh->hist[b];
                 }
             }
-            const float s_cell = fminf(smax, sbase * exp2f((float)B));
+            const float s_cell = uni(fminf(smax, sbase * exp2f((float)B)));
             if (tid == 0) sh->nout = 0u;
             __syncthreads();
             for (uint32_t c = tid; c < Nc; c += WG)
                 if (w.skin[c] > s_cell) sh->out[atomicAdd(&sh->nout, 1u)] = (uint16_t)c;
             __syncthreads();
-            const uint32_t n_out = sh->nout;
-            const double h = 2.0 * ((double)rmax + (double)s_cell) * 1.001 + 1e-30;
+            const uint32_t n_out = uni(sh->nout);
+            const double h = cell_size(rmax, s_cell);
             const double inv_h = 1.0 / h;
             for (uint32_t b = tid; b <= T; b += WG) w.start[b] = 0u;
             for (uint32_t b = tid; b < T; b += WG) w.fill[b] = 0u;
@@ -311,9 +522,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             for (uint32_t c = tid; c < Nc; c += WG) {
                 float2 p = P[Np + c];
                 w.ref[c] = p;
-                int cx = scell(p.x, inv_h), cy = scell(p.y, inv_h);
-                w.cell[c] = make_int2(cx, cy);
-                uint32_t kk = cell_hash(cx, cy) & tmask;
+                uint32_t kk = cell_hash(scell(p.x, inv_h), scell(p.y, inv_h)) & tmask;
                 w.key[c] = kk;
                 atomicAdd(&w.start[kk], 1u);
             }
@@ -321,25 +530,30 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             wg_scan_global<WG>(w.start, T + 1, s_u32);  // bucket k = [start[k], start[k+1])
             for (uint32_t c = tid; c < Nc; c += WG) {
                 uint32_t kk = w.key[c];
-                w.items[w.start[kk] + atomicAdd(&w.fill[kk], 1u)] = c;
+                float2 p = w.ref[c];
+                float sk = w.skin[c];
+                w.rec[w.start[kk] + atomicAdd(&w.fill[kk], 1u)] =
+                    make_float4(p.x, p.y, crad[c] + sk, __uint_as_float(c | (sk > s_cell ? kOutlierBit : 0u)));
             }
             __syncthreads();
             // counts: normal circles their normal partners (kept in stamp until the
             // sweep needs it), then outliers add themselves to their partners
             for (uint32_t c = tid; c < Nc; c += WG) {
-                uint32_t n = w.skin[c] > s_cell ? 0u : gather_normal(c, w, crad, s_cell, tmask, nullptr);
+                const float sk = w.skin[c];
+                uint32_t n = sk > s_cell ? 0u : gather_normal(c, w, w.ref[c], crad[c] + sk, inv_h, tmask, nullptr);
                 w.cnt[c] = n;
                 w.stamp[c] = n;
             }
             __syncthreads();
-            for (uint32_t k = 0; k < n_out; k++)
-                gather_outlier<WG, 0>(sh->out[k], Nc, w, crad, rmax, s_cell, smax, h, tmask, nullptr);
+            const uint32_t n_far =
+                gather_outliers<WG, 0>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, nullptr);
             __syncthreads();
+            const uint64_t tb1 = wall_clock64();
             for (uint32_t c = tid; c < Nc; c += WG) w.off[c] = w.cnt[c];
             if (tid == 0) w.off[Nc] = 0u;
             __syncthreads();
             wg_scan_global<WG>(w.off, Nc + 1, s_u32);
-            const uint32_t total = w.off[Nc];
+            const uint32_t total = uni(w.off[Nc]);
             if (total > kCandCap * Nc) {
                 // more candidates than the workspace holds: smaller skins
                 __syncthreads();
@@ -352,33 +566,71 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             }
             // lists: normal partners first, outliers append at a cursor (in head)
             for (uint32_t c = tid; c < Nc; c += WG) {
+                const float sk = w.skin[c];
                 uint32_t own = 0;
-                if (w.skin[c] <= s_cell) own = gather_normal(c, w, crad, s_cell, tmask, w.cand + w.off[c]);
+                if (sk <= s_cell) own = gather_normal(c, w, w.ref[c], crad[c] + sk, inv_h, tmask, w.cand + w.off[c]);
                 w.head[c] = w.off[c] + own;
             }
             __syncthreads();
-            for (uint32_t k = 0; k < n_out; k++)
-                gather_outlier<WG, 1>(sh->out[k], Nc, w, crad, rmax, s_cell, smax, h, tmask, w.head);
+            gather_outliers<WG, 1>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, w.head);
             __syncthreads();
+            const uint64_t tb2 = wall_clock64();
+            // ascending partner order (the reference's pair order): short lists by
+            // their thread, longer ones by a wavefront, the longest by the workgroup
+            // (queued at the two ends of wl)
+            if (tid == 0) { sh->nmid = 0u; sh->nlong = 0u; }
+            __syncthreads();
+            const uint32_t wl_end = 2u * (Nc / 2u + 1u) - 1u;
+            uint32_t longest = 0;
             for (uint32_t c = tid; c < Nc; c += WG) {
                 uint16_t *my = w.cand + w.off[c];
                 const uint32_t n = w.cnt[c];
-                for (uint32_t x = 1; x < n; x++) {  // ascending: the reference's pair order
-                    uint16_t v = my[x];
-                    uint32_t y = x;
-                    while (y > 0 && my[y - 1] > v) { my[y] = my[y - 1]; y--; }
-                    my[y] = v;
-                }
+                longest = max(longest, n);
+                if (n <= 1u) continue;
+                if (n <= 8u) sort_in_registers<8>(my, n);
+                else if (n <= 16u) sort_in_registers<16>(my, n);
+                else if (n <= 32u) sort_in_registers<32>(my, n);
+                else if (n <= kMidList) w.wl[atomicAdd(&sh->nmid, 1u)] = c;
+                else w.wl[wl_end - atomicAdd(&sh->nlong, 1u)] = c;
             }
-            // base skin: rebuilds every few sweeps cost more than larger lists
-            if (since_build < 4) sbase = fminf(1.25f * sbase, 0.5f * fmaxf(rmax, 1e-3f));
-            else if (since_build > 32) sbase = fmaxf(0.8f * sbase, 0.05f * fmaxf(rmax, 1e-3f));
-            since_build = 0;
-            list_valid = true;
-            n_pairs = total / 2;
-            u = make_float2(0.0f, 0.0f);
             __syncthreads();
-            t_build += wall_clock64() - tb0;
+            const uint32_t nmid = uni(sh->nmid), nlong = uni(sh->nlong);
+            for (uint32_t q = tid >> 6; q < nmid; q += WG / 64) {
+                const uint32_t c = w.wl[q];
+                sort_by_wave(w.cand + w.off[c], w.cnt[c]);
+            }
+            for (uint32_t q = 0; q < nlong; q++) {
+                const uint32_t c = w.wl[wl_end - q], n = w.cnt[c];
+                uint16_t *my = w.cand + w.off[c];
+                for (uint32_t k = tid; k < n; k += WG) w.items[k] = my[k];
+                __syncthreads();
+                // partners are distinct, so the ranks are a permutation of [0, n)
+                for (uint32_t k = tid; k < n; k += WG) {
+                    const uint32_t v = w.items[k];
+                    uint32_t rank = 0;
+                    for (uint32_t j2 = 0; j2 < n; j2++) rank += w.items[j2] < v ? 1u : 0u;
+                    my[rank] = (uint16_t)v;
+                }
+                __syncthreads();
+            }
+            __syncthreads();
+            // base skin: rebuilds every few sweeps cost more than larger lists
+            if (cs.since_build < 4) sbase = fminf(1.25f * sbase, 0.5f * fmaxf(rmax, 1e-3f));
+            else if (cs.since_build > 32) sbase = fmaxf(0.8f * sbase, 0.05f * fmaxf(rmax, 1e-3f));
+            cs.since_build = 0;
+            list_valid = true;
+            cs.pairs = total / 2;
+            cs.n_out = n_out;
+            cs.n_far = n_far;
+            cs.s_cell = s_cell;
+            cs.smax = smax;
+            u = make_float2(0.0f, 0.0f);
+            cs.max_list = (uint32_t)wg_max<WG>((float)longest, s_red);
+            const uint64_t tb3 = wall_clock64();
+            cs.t_build += tb3 - tb0;
+            cs.t_count += tb1 - tb0;
+            cs.t_lists += tb2 - tb1;
+            cs.t_sort += tb3 - tb2;
         }
         // ---- sweep: worklist rounds
         for (uint32_t c = tid; c < Nc; c += WG) {
@@ -396,22 +648,25 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             if (c < o && w.cand[w.off[o]] == c) wl0[atomicAdd(&sh->cnt3[0], 1u)] = c | (o << 16);
         }
         __syncthreads();
-        uint32_t n = sh->cnt3[0];
+        uint32_t n = uni(sh->cnt3[0]);
         for (uint32_t r = 0; n > 0; r++) {
             uint32_t *cur = (r & 1) ? wl1 : wl0, *nxt = (r & 1) ? wl0 : wl1;
             // apply the ready pairs and move both heads
             for (uint32_t k = tid; k < n; k += WG) {
                 uint32_t pr = cur[k], c = pr & 0xffffu, o = pr >> 16;
                 float2 pi = P[Np + c], pj = P[Np + o];
+                const uint32_t hc = w.head[c], ho = w.head[o];
                 float qi = 0.0f, qj = 0.0f;
                 if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, crad[c], crad[o], qi, qj)) {
                     P[Np + c] = pi;
                     P[Np + o] = pj;
-                    w.path[c] = w.path[c] + qi;
-                    w.path[o] = w.path[o] + qj;
+                    w.path[c] = excursion(w.path[c], pi, w.save[c]);
+                    w.path[o] = excursion(w.path[o], pj, w.save[o]);
                 }
-                w.head[c] = w.head[c] + 1u;
-                w.head[o] = w.head[o] + 1u;
+                w.traj[w.off[c] + hc] = pi;
+                w.traj[w.off[o] + ho] = pj;
+                w.head[c] = hc + 1u;
+                w.head[o] = ho + 1u;
                 w.stamp[c] = r;
                 w.stamp[o] = r;
             }
@@ -433,35 +688,57 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 }
             }
             __syncthreads();
-            n = sh->cnt3[(r + 1) % 3];
-            n_rounds++;
+            n = uni(sh->cnt3[(r + 1) % 3]);
+            cs.rounds++;
         }
-        // every list must be exhausted (a schedule that stalls is an error)
-        float left = 0.0f, fail = 0.0f;
+        // every list must be exhausted (a schedule that stalls is an error);
+        // circles beyond 0.8 of their skin go to out[] (free after list builds)
+        // and vb (stamp, free after the rounds) gets each circle's bound v,
+        // negated for those
+        float left = 0.0f, vmax = 0.0f;
+        float *vb = (float *)w.stamp;
+        if (tid == 0) sh->nout = 0u;
+        __syncthreads();
         for (uint32_t c = tid; c < Nc; c += WG) {
             if (w.head[c] != w.cnt[c]) left = 1.0f;
             float pth = w.path[c];
             float v = moved_bound(w.save[c], w.ref[c], u, pth);
             // the sweep's own pushes, decaying slowly so circles in lasting contact keep their skin
             w.need[c] = (pth <= 3.0e38f) ? fmaxf(2.5f * pth, 0.75f * w.need[c]) : 0.0f;
-            if (!(v <= 0.8f * w.skin[c])) fail = 1.0f;
+            if (v <= 0.8f * w.skin[c]) {
+                vmax = fmaxf(vmax, v);
+                vb[c] = v;
+            } else {
+                vb[c] = -v;
+                uint32_t k = atomicAdd(&sh->nout, 1u);
+                if (k < kOutCap) sh->out[k] = (uint16_t)c;
+            }
         }
         if (wg_max<WG>(left, s_red) > 0.0f) return 3u;
-        if (wg_max<WG>(fail, s_red) == 0.0f) {
-            since_build++;
+        const uint32_t n_over = uni(sh->nout);
+        vmax = wg_max<WG>(vmax, s_red);
+        const double h = cell_size(rmax, cs.s_cell);  // the grid of the list build
+        if (n_over == 0u || (n_over <= kOutCap && overruns_safe<WG>(w, crad, vb, sh->out, n_over, vmax, rmax, h,
+                                                                   1.0 / h, tmask, s_red))) {
+            cs.since_build++;
+            if (n_over) cs.kept++;
             return 0u;
         }
         // undo and rebuild with the skins the circles needed
         for (uint32_t c = tid; c < Nc; c += WG) P[Np + c] = w.save[c];
         __syncthreads();
-        n_redos++;
+        cs.redos++;
         list_valid = false;
     }
 }
 
 template <int WG, int MODE>
-__device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, float2 *s_lds, ColliderStage *s_col,
+__device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint32_t steps, SceneState *st, float2 *s_lds,
+                          ColliderStage *s_col,
                           float *s_red, uint32_t *s_u32, CCShared *s_cc) {
+    // fields are read where used: the descriptor is uniform and read-only, so
+    // they come through scalar loads instead of a private copy
+    const SceneDesc &d = *dp;
     char *base = (char *)d.base;
     float2 *gpos = (float2 *)(base + d.pos_off);
     float2 *gprev = (float2 *)(base + d.prev_off);
@@ -501,20 +778,24 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
     w.ref = (float2 *)(ws + cl.ref); w.save = (float2 *)(ws + cl.save);
     w.path = (float *)(ws + cl.path); w.skin = (float *)(ws + cl.skin); w.need = (float *)(ws + cl.need);
     w.cnt = (uint32_t *)(ws + cl.cnt); w.off = (uint32_t *)(ws + cl.off); w.cand = (uint16_t *)(ws + cl.cand);
+    w.traj = (float2 *)(ws + cl.traj);
     w.head = (uint32_t *)(ws + cl.head); w.stamp = (uint32_t *)(ws + cl.stamp); w.wl = (uint32_t *)(ws + cl.wl);
-    w.key = (uint32_t *)(ws + cl.key); w.cell = (int2 *)(ws + cl.cell); w.start = (uint32_t *)(ws + cl.start);
+    w.key = (uint32_t *)(ws + cl.key); w.rec = (float4 *)(ws + cl.rec); w.start = (uint32_t *)(ws + cl.start);
     w.fill = (uint32_t *)(ws + cl.fill); w.items = (uint32_t *)(ws + cl.items);
     if (Nc >= 2 && !st->ws_ready) {
         for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = 0.0f;
         __syncthreads();
     }
 
-    float skin = st->skin;
-    bool list_valid = st->list_valid != 0;
-    uint32_t err = st->err;
-    uint32_t n_rounds = 0, n_redos = 0, n_rebuilds = 0, n_pairs = st->pairs;
-    uint64_t t_links = 0, t_build = 0, t_sweep = 0, t_bodies = 0;
-    uint32_t since_build = st->since_build;
+    float skin = uni(st->skin);
+    bool list_valid = uni(st->list_valid) != 0;
+    uint32_t err = uni(st->err);
+    CCStats cs{};
+    cs.pairs = uni(st->pairs);
+    cs.since_build = uni(st->since_build);
+    cs.n_out = uni(st->n_out); cs.n_far = uni(st->n_far); cs.max_list = uni(st->max_list);
+    cs.s_cell = uni(st->s_cell); cs.smax = uni(st->smax);
+    uint64_t t_links = 0, t_sweep = 0, t_bodies = 0;
     bool acc = (d.flags & kFlagAcc) != 0;
     const float2 *acc0 = (const float2 *)(base + d.acc0_off);
 
@@ -546,8 +827,7 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
             if (Nc >= 2 && !err)
             {
                 const uint64_t t0 = wall_clock64();
-                err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, skin, list_valid,
-                                       n_rounds, n_redos, n_rebuilds, n_pairs, t_build, since_build);
+                err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, skin, list_valid, cs);
                 t_sweep += wall_clock64() - t0;
             }
             // ---- bodies: colliders -> bounds -> Verlet
@@ -593,16 +873,25 @@ __device__ void run_scene(const SceneDesc &d, uint32_t steps, SceneState *st, fl
         st->skin = skin;
         st->list_valid = list_valid ? 1u : 0u;
         st->err = err;
-        st->rounds += n_rounds;
-        st->redos += n_redos;
-        st->rebuilds += n_rebuilds;
-        st->pairs = n_pairs;
+        st->rounds += cs.rounds;
+        st->redos += cs.redos;
+        st->rebuilds += cs.rebuilds;
+        st->pairs = cs.pairs;
         if (Nc >= 2) st->ws_ready = 1u;
         st->t_links += t_links;
-        st->t_build += t_build;
-        st->t_sweep += t_sweep - t_build;
+        st->t_build += cs.t_build;
+        st->t_sweep += t_sweep - cs.t_build;
         st->t_bodies += t_bodies;
-        st->since_build = since_build;
+        st->since_build = cs.since_build;
+        st->n_out = cs.n_out;
+        st->n_far = cs.n_far;
+        st->max_list = cs.max_list;
+        st->s_cell = cs.s_cell;
+        st->smax = cs.smax;
+        st->t_count += cs.t_count;
+        st->t_lists += cs.t_lists;
+        st->t_sort += cs.t_sort;
+        st->kept += cs.kept;
     }
     __syncthreads();
 }
@@ -620,19 +909,19 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
     for (;;) {
         if (threadIdx.x == 0) s_next = atomicAdd(queue, 1u);
         __syncthreads();
-        uint32_t k = s_next;
+        const uint32_t k = uni(s_next);
         __syncthreads();
         if (k >= n) break;
-        uint32_t si = order[k];
-        const SceneDesc d = descs[si];
-        const uint32_t steps = d.steps < launch_steps ? d.steps : launch_steps;
-        if (steps == 0 || d.N == 0) continue;
-        if (16ull * d.N <= lds_bytes)
-            run_scene<WG, 2>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
-        else if (8ull * d.N <= lds_bytes)
-            run_scene<WG, 1>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
+        const uint32_t si = uni(order[k]);
+        const SceneDesc *__restrict__ d = descs + si;
+        const uint32_t steps = d->steps < launch_steps ? d->steps : launch_steps;
+        if (steps == 0 || d->N == 0) continue;
+        if (16ull * d->N <= lds_bytes)
+            run_scene<WG, 2>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
+        else if (8ull * d->N <= lds_bytes)
+            run_scene<WG, 1>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
         else
-            run_scene<WG, 0>(d, steps, states + d.state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
+            run_scene<WG, 0>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
     }
 }
 
@@ -890,7 +1179,7 @@ private:
     SceneState *state_ptr() { return (SceneState *)(dev_ + 384); }
     hipStream_t stream_;
     uint8_t *chunk_ = nullptr;
-    uint8_t *dev_ = nullptr;  // [0,256) desc, [256,260) order, [260,264) queue, [384,416) state
+    uint8_t *dev_ = nullptr;  // [0,256) desc, [256,260) order, [260,264) queue, [384,512) state
     uint8_t host_blob_[264];
     SceneDesc desc_{};
     bool acc_pending_ = false;

@@ -59,14 +59,20 @@ struct SceneState {
     uint32_t rebuilds;   // candidate lists built
     uint32_t pairs;      // candidate pairs in the last list
     uint32_t ws_ready;   // per-circle skin state initialised
-    uint64_t t_links, t_build, t_sweep, t_bodies;  // phase times, 100 MHz ticks (statistics)
+    uint64_t t_links, t_build, t_sweep, t_bodies;  // phase times, device wall-clock ticks (statistics)
     uint32_t since_build;  // sweeps since the candidate list was built
-    uint32_t pad2;
+    // the last candidate list (statistics): outliers, of them searched against
+    // every circle, longest list, outlier skin threshold, largest skin
+    uint32_t n_out, n_far, max_list;
+    float s_cell, smax;
+    uint64_t t_count, t_lists, t_sort;  // list build split: counts, lists, sorting (ticks)
+    uint32_t kept;  // sweeps kept although circles left their skins (no missed pair touched)
 };
+static_assert(sizeof(SceneState) <= 128, "SceneState must fit the single-scene device blob");
 
 // Byte offsets of the circle-pair workspace inside ws (relative to ws_off).
 struct CCLayout {
-    uint32_t ref, save, path, skin, need, cnt, off, cand, head, stamp, wl, key, cell, start, fill, items, bytes;
+    uint32_t ref, save, path, skin, need, cnt, off, cand, traj, head, stamp, wl, key, rec, start, fill, items, bytes;
 };
 
 BD_HD uint32_t align16(uint64_t v) { return (uint32_t)((v + 15) & ~uint64_t(15)); }
@@ -82,11 +88,12 @@ BD_HD CCLayout cc_layout(uint32_t Nc, uint32_t T) {
     c.cnt = (uint32_t)o; o = align16(o + 4ull * Nc);
     c.off = (uint32_t)o; o = align16(o + 4ull * (Nc + 1));
     c.cand = (uint32_t)o; o = align16(o + 2ull * kCandCap * Nc);
+    c.traj = (uint32_t)o; o = align16(o + 8ull * kCandCap * Nc);
     c.head = (uint32_t)o; o = align16(o + 4ull * Nc);
     c.stamp = (uint32_t)o; o = align16(o + 4ull * Nc);
     c.wl = (uint32_t)o; o = align16(o + 4ull * 2 * (Nc / 2 + 1));
     c.key = (uint32_t)o; o = align16(o + 4ull * Nc);
-    c.cell = (uint32_t)o; o = align16(o + 8ull * Nc);
+    c.rec = (uint32_t)o; o = align16(o + 16ull * Nc);
     c.start = (uint32_t)o; o = align16(o + 4ull * (T + 1));
     c.fill = (uint32_t)o; o = align16(o + 4ull * T);
     c.items = (uint32_t)o; o = align16(o + 4ull * Nc);

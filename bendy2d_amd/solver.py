@@ -212,13 +212,15 @@ class Solver:
     def get_particles(self):
         n = self.get_particle_len()
         out = np.zeros((max(n, 1), 6), np.float32)
-        lib().bd_get_particles(self._h, _ptr(out), n)
+        if lib().bd_get_particles(self._h, _ptr(out), n) != n:
+            raise RuntimeError("bd_get_particles: " + lib().bd_last_error().decode())
         return out[:n]
 
     def get_circles(self):
         n = self.get_circles_len()
         out = np.zeros((max(n, 1), 7), np.float32)
-        lib().bd_get_circles(self._h, _ptr(out), n)
+        if lib().bd_get_circles(self._h, _ptr(out), n) != n:
+            raise RuntimeError("bd_get_circles: " + lib().bd_last_error().decode())
         return out[:n]
 
     def get_particle(self, i):
@@ -352,6 +354,16 @@ class Batch:
         v = [ctypes.c_double() for _ in range(4)]
         check(lib().bd_batch_scene_times(self._h, int(i), *[ctypes.byref(x) for x in v]), "batch_scene_times")
         return dict(zip(("links", "build", "sweep", "bodies"), [x.value for x in v]))
+
+    def scene_lists(self, i):
+        u = [ctypes.c_uint32() for _ in range(3)]
+        f = [ctypes.c_float() for _ in range(2)]
+        d = [ctypes.c_double() for _ in range(3)]
+        k = [ctypes.c_uint32()]
+        check(lib().bd_batch_scene_lists(self._h, int(i), *[ctypes.byref(x) for x in u + f + d + k]),
+              "batch_scene_lists")
+        names = ("n_out", "n_far", "max_list", "s_cell", "smax", "t_count", "t_lists", "t_sort", "kept")
+        return dict(zip(names, [x.value for x in u + f + d + k]))
 
     def cc_stats(self):
         v = [ctypes.c_uint64() for _ in range(4)]

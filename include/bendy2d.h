@@ -319,6 +319,15 @@ int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *er
 /* Device time a scene spent per phase so far, in seconds: link layers,
  * candidate-list builds, pair sweeps (rounds and checks), bodies. */
 int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *build, double *sweep, double *bodies);
+/* The scene's last candidate-list build (statistics): outlier circles (skin
+ * above the cell threshold), of them those searched against every circle,
+ * the longest list, the threshold and the largest skin; the device time so
+ * far of list builds split into counting, list filling and sorting, in
+ * seconds; and the sweeps kept although circles moved beyond their skins
+ * (every pair missing from the lists was checked not to touch). */
+int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t *n_far, uint32_t *max_list,
+                             float *s_cell, float *smax, double *t_count, double *t_lists, double *t_sort,
+                             uint32_t *kept);
 
 /* ------------------------------------------------------------------ */
 /* Serving: continuous batching of scene-advance requests              */

@@ -108,3 +108,48 @@ def test_batch_large_scene_global_residency():
         for i, x in enumerate(cl):
             b.read_scene(i, x)
             assert bits_equal(state(x), want[i]), f"lds {lds} scene {i}: {first_diff(state(x), want[i])}"
+
+
+def circle_column(wide, tall, seed):
+    """wide x tall free circles (r 0.35 at spacing 0.8) stacked in a box one
+    column wide, 8 substeps: the stack settles into lasting dense contact,
+    where circles keep pushing each other around within every sweep."""
+    s = Solver()
+    rng = np.random.default_rng(seed)
+    for r in range(tall):
+        for c in range(wide):
+            s.add_circle((1.0 + 0.8 * c + rng.uniform(-0.05, 0.05), 1.0 + 0.8 * r), 0.35)
+    s.gravity = (0.0, -9.81)
+    s.bounds = ((0.0, 0.0), (0.8 * wide + 1.2, 0.8 * tall + 10.0))
+    s.set_sub_steps(8)
+    s.set_link_iterations(1)
+    return s
+
+
+def test_batch_dense_contact_exact():
+    # in dense contact, circles leave their skins in many sweeps; such a sweep
+    # is kept only when every pair missing from the lists provably did not
+    # touch at its turn, so the result must stay bit-identical to the
+    # oracle's all-pairs loop
+    scenes = [circle_column(12, 120, 1), circle_column(10, 140, 2)]
+    b = Batch()
+    for s in scenes:
+        b.add_scene(s)
+    oracles = [to_oracle(s, 32) for s in scenes]
+    done = 0
+    for chunk in (60, 60, 60, 60):
+        b.update(DT, chunk)
+        b.sync()
+        for o in oracles:
+            for _ in range(chunk):
+                assert o.update(DT) == 0
+        done += chunk
+        for i, s in enumerate(scenes):
+            b.read_scene(i, s)
+            a, w = state(s), state(oracles[i])
+            assert bits_equal(a, w), f"scene {i} step {done}: first diff {first_diff(a, w)}"
+    st = [b.scene_status(i) for i in range(len(scenes))]
+    assert all(x["err"] == 0 for x in st)
+    assert sum(x["rounds"] for x in st) > 0
+    kept = sum(b.scene_lists(i)["kept"] for i in range(len(scenes)))
+    print("sweeps kept past skins:", kept, "redos:", sum(x["redos"] for x in st))
