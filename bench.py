@@ -43,7 +43,9 @@ def build_softbody_scenes(n, seed0=1000, max_side=0):
     return scenes, pts, links
 
 
-def bench_batched(scenes, pts, links, steps, warmup, tune=True):
+def bench_batched(scenes, pts, links, steps, warmup, tune=True, launch=None):
+    """Times `steps` updates of the whole batch (rounded up to whole launches of
+    steps_per_launch updates) after `warmup` untimed ones."""
     from bendy2d_amd import Batch
     b = Batch()
     t0 = time.time()
@@ -52,10 +54,14 @@ def bench_batched(scenes, pts, links, steps, warmup, tune=True):
     b.update(DT, 0)  # upload
     b.sync()
     log(f"batch: {len(scenes)} scenes, {pts} bodies, {links} links, upload {time.time() - t0:.1f}s")
-    if tune:
+    if launch:
+        b.config = launch
+    elif tune:
         t0 = time.time()
         tried = b.autotune(DT, 2)
         log(f"autotune: {tried} configs in {time.time() - t0:.1f}s -> {b.config}")
+    spl = max(1, int(b.config["steps_per_launch"]))
+    steps = -(-steps // spl) * spl
     b.update(DT, warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -63,14 +69,16 @@ def bench_batched(scenes, pts, links, steps, warmup, tune=True):
     b.sync()
     wall = time.perf_counter() - t0
     dev_ms, launches = b.last_timing()
+    replays = steps // spl
     cc = b.cc_stats()
     substeps = 8
     pu = pts * substeps * steps
     # bytes per substep: 64 B per body (integrate + collision) + 44 B per link
     alg_bytes = (64.0 * pts + 44.0 * links) * substeps * steps
     return {
-        "pu_per_s": pu / wall, "wall_s": wall, "device_ms": dev_ms, "launches": launches,
-        "alg_bytes": alg_bytes, "pu": pu, "config": b.config, "cc": cc, "batch": b,
+        "pu_per_s": pu / wall, "wall_s": wall, "device_ms": dev_ms, "kernel_launches": launches,
+        "replays": replays, "steps": steps, "alg_bytes": alg_bytes, "pu": pu, "config": b.config, "cc": cc,
+        "batch": b,
     }
 
 
@@ -113,27 +121,29 @@ def bench_cloth(n, steps, warmup):
             "bytes_per_pu": alg / pu}
 
 
-def bench_serving(seconds, rate, launch_cfg, steps_per_launch=8, budget=64 << 30, seed=4):
+def bench_serving(seconds, rate, launch_cfg, steps_per_launch=8, budget=64 << 30, seed=4, drain_limit=30.0,
+                  launch_budget_us=20000):
     """Poisson stream of requests: half create a new softbody scene (generated
     as in the batched workload, seeded by request number), half continue a
     uniformly chosen known scene; each advances its scene K ~ U{10..600}
-    updates.  Arrivals stop after `seconds`; the server then drains."""
-    import threading
+    updates.  Arrivals stop after `seconds`; the server then drains for at
+    most `drain_limit` seconds.  Latency percentiles cover every submitted
+    request: one still pending at the end counts with its age then, a lower
+    bound on its latency (reported as censored)."""
     import numpy as np
     from bendy2d_amd import Server, Solver, workloads
     rng = np.random.default_rng(seed)
     n = int(rate * seconds * 1.2) + 16
-    gaps = rng.exponential(1.0 / rate, n)
-    arrivals = np.cumsum(gaps)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
     arrivals = arrivals[arrivals < seconds]
     kinds = rng.random(len(arrivals)) < 0.5       # True: create
     ks = rng.integers(10, 601, len(arrivals))
     picks = rng.random(len(arrivals))
-    srv = Server(memory_budget=budget, steps_per_launch=steps_per_launch,
+    srv = Server(memory_budget=budget, steps_per_launch=steps_per_launch, launch_budget_us=launch_budget_us,
                  launch={k: launch_cfg[k] for k in ("wg_small", "wg_large", "split_points", "lds_small", "lds_large")})
-    known = []
-    lag = []
+    known, lag, submitted, done = [], [], {}, []
     t0 = time.perf_counter()
+    last_log = 0.0
     for i, ta in enumerate(arrivals):
         now = time.perf_counter() - t0
         if ta > now:
@@ -142,30 +152,68 @@ def bench_serving(seconds, rate, launch_cfg, steps_per_launch=8, budget=64 << 30
         if kinds[i] or not known:
             s = Solver()
             workloads.softbody(s, 500000 + i)
-            known.append(srv.create(s, int(ks[i]), DT)[1])
+            rid, sid = srv.create(s, int(ks[i]), DT)
+            known.append(sid)
             s.close()
         else:
-            srv.advance(known[int(picks[i] * len(known))], int(ks[i]), DT)
+            rid = srv.advance(known[int(picks[i] * len(known))], int(ks[i]), DT)
+        submitted[rid] = srv.now()
+        if ta - last_log >= 5.0:
+            done += srv.poll()
+            st = srv.stats()
+            log(f"serving t={ta:.0f}s submitted={i + 1} done={len(done)} pending={st['requests_pending']} "
+                f"resident={st['resident_scenes']} GB={st['bytes_used'] / 2**30:.1f} busy={st['busy_s']:.1f}s")
+            last_log = ta
     t_arrive_end = time.perf_counter() - t0
-    srv.wait_idle(600)
+    while True:
+        idle = srv.wait_idle(5.0)
+        done += srv.poll()
+        el = time.perf_counter() - t0
+        st = srv.stats()
+        log(f"serving drain t={el:.0f}s done={len(done)} pending={st['requests_pending']} busy={st['busy_s']:.1f}s")
+        if idle or el - t_arrive_end >= drain_limit:
+            break
     t_end = time.perf_counter() - t0
-    done = srv.poll()
+    t_end_srv = srv.now()
     st = srv.stats()
     srv.close()
-    lat = np.array([d["t_done"] - d["t_submit"] for d in done])
+    finished = {d["request_id"]: d for d in done}
+    lat = np.array([finished[r]["t_done"] - finished[r]["t_submit"] if r in finished else t_end_srv - ts
+                    for r, ts in submitted.items()])
+    censored = len(submitted) - len(finished)
     pu = sum(d["particle_updates"] for d in done)
-    bad = sum(1 for d in done if d["status"] != 0)
+
+    def pct(q):
+        v = float(np.percentile(lat, q))
+        # a percentile at or above the smallest censored age is only a lower bound
+        lower = censored > 0 and v >= min(t_end_srv - submitted[r] for r in submitted if r not in finished)
+        return v * 1e3, lower
+    p50, p50_lb = pct(50)
+    p99, p99_lb = pct(99)
     return {
-        "requests": len(done), "rate_req_per_s": rate, "arrival_seconds": seconds,
-        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3),
+        "requests": len(submitted), "completed": len(finished), "censored": censored,
+        "failed": sum(1 for d in done if d["status"] != 0),
+        "rate_req_per_s": rate, "arrival_seconds": seconds, "drain_seconds": t_end - t_arrive_end,
+        "p50_ms": p50, "p99_ms": p99, "p50_is_lower_bound": p50_lb, "p99_is_lower_bound": p99_lb,
         "pu_per_s": pu / t_end, "pu_per_s_device_busy": st["particle_updates"] / max(st["busy_s"], 1e-9),
-        "device_busy_frac": st["busy_s"] / t_end, "drain_s": t_end - t_arrive_end,
-        "launches": st["launches"], "resident_scenes": st["resident_scenes"], "bytes_used": st["bytes_used"],
-        "evictions": st["evictions"], "failed": bad,
-        "driver_lag_ms_p99": float(np.percentile(np.array(lag), 99) * 1e3),
-        "steps_per_launch": steps_per_launch,
+        "device_busy_frac": st["busy_s"] / t_end, "launches": st["launches"],
+        "resident_scenes": st["resident_scenes"], "bytes_used": st["bytes_used"], "evictions": st["evictions"],
+        "driver_lag_ms_p99": float(np.percentile(np.array(lag), 99) * 1e3), "steps_per_launch": steps_per_launch,
+        "launch_budget_us": launch_budget_us,
     }
+
+
+def copy_calibration(nbytes=1 << 30):
+    """One plain device copy of nbytes (torch), for scaling the HBM byte
+    counters of a profiler run: its FETCH_SIZE and WRITE_SIZE should each read
+    nbytes."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    return nbytes
 
 
 def cpu_baseline(seconds=12.0, seed0=1000):
@@ -191,18 +239,33 @@ def cpu_baseline(seconds=12.0, seed0=1000):
                       f"4096-scene workload, 1 update (8 substeps) each, {busy:.1f} s CPU"}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100, help="timed updates of the batch (rounded up to whole launches)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--cloth", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=0, help="scenes for the plain path (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-drain", type=float, default=30.0, help="longest wait for the serving backlog")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--launch", type=str, default="", help="batch launch config as JSON (skips the autotuner)")
+    ap.add_argument("--only", choices=("all", "batched", "cloth", "copy"), default="all",
+                    help="run one part alone (for profiler runs); only 'all' prints the JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box of this run has one MI355X; --gpus {args.gpus} is not run. "
@@ -212,12 +275,24 @@ def main():
     if lib().bd_device_count() < 1:
         log("no HIP device visible")
         sys.exit(1)
+    launch = json.loads(args.launch) if args.launch else None
+
+    if args.only == "copy":
+        log(f"copy calibration: {copy_calibration()} bytes")
+        return
+    if args.only == "cloth":
+        log(f"cloth {args.cloth}^2: {bench_cloth(args.cloth, args.steps, args.warmup)}")
+        return
 
     t0 = time.time()
     scenes, pts, links = build_softbody_scenes(args.scenes)
     log(f"built {args.scenes} scenes in {time.time() - t0:.1f}s")
-    bat = bench_batched(scenes, pts, links, args.steps, args.warmup, tune=not args.no_tune)
-    log(f"batched: {bat['pu_per_s']:.3e} PU/s, {bat['wall_s'] * 1e3 / args.steps:.3f} ms/step, cc {bat['cc']}")
+    bat = bench_batched(scenes, pts, links, args.steps, args.warmup, tune=not args.no_tune, launch=launch)
+    log(f"batched: {bat['pu_per_s']:.3e} PU/s, {bat['wall_s'] * 1e3 / bat['steps']:.3f} ms/step, "
+        f"device {bat['device_ms']:.1f} ms over {bat['replays']} launches, cc {bat['cc']}")
+    if args.only == "batched":
+        log(f"copy calibration: {copy_calibration()} bytes")
+        return
 
     sample = scenes if not args.plain_sample else scenes[::max(1, len(scenes) // args.plain_sample)]
     spts = sum(s.stats()["n_points"] for s in sample)
@@ -234,25 +309,29 @@ def main():
         bat["batch"].close()
         for s in scenes:
             s.close()
-        serving = bench_serving(args.serve_seconds, args.serve_rate, bat["config"])
+        serving = bench_serving(args.serve_seconds, args.serve_rate, bat["config"], drain_limit=args.serve_drain)
         log(f"serving: {serving}")
 
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
+        cpu["cpu_model"] = cpu_model()
+        cpu["host_cpus"] = os.cpu_count()
         log(f"cpu: {cpu}")
 
-    launches = max(bat["launches"], 1)
-    graph_launches = max(1, launches // 3) if bat["config"]["use_graph"] else launches
-    per_launch_s = bat["device_ms"] / 1e3 / graph_launches
-    achieved = bat["alg_bytes"] / graph_launches / per_launch_s
+    # dominant kernel: one launch (graph replay) of k_scenes, both size classes
+    # running concurrently on their streams, timed by HIP events on the batch's
+    # stream around all timed launches
+    per_launch_s = bat["device_ms"] / 1e3 / max(bat["replays"], 1)
+    bytes_per_launch = bat["alg_bytes"] / max(bat["replays"], 1)
+    achieved = bytes_per_launch / per_launch_s
     out = {
         "metric": "particle-updates/s batched vs plain (1 GPU) + p50/p99 request latency",
         "value": bat["pu_per_s"],
         "unit": "particle-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": bat["steps"],
         "warmup": args.warmup,
-        "ms_per_step": bat["wall_s"] * 1e3 / args.steps,
+        "ms_per_step": bat["wall_s"] * 1e3 / bat["steps"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -262,14 +341,16 @@ def main():
             "workload": f"{args.scenes} independent softbody scenes (grid of circles r=0.35, side log-uniform "
                         "4..128, structural+shear circle links, 1-8 static circle/convex-polygon colliders, "
                         "circle-pair collision on), 8 substeps and 1 link sweep per update, batched persistent "
-                        "launch replayed as a hipGraph",
+                        "launch replayed as a hipGraph; updates 1.." + str(args.warmup + bat["steps"]) + " from rest",
             "scenes": args.scenes, "bodies": pts, "links": links, "substeps": 8,
             "launch": {k: v for k, v in bat["config"].items()},
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "k_scenes (both size classes, concurrent) per graph launch",
+            "kernel": "k_scenes, one launch = both size classes concurrently, "
+                      f"{bat['config']['steps_per_launch']} updates; {bytes_per_launch / 1e9:.3f} GB algorithmic "
+                      f"per launch in {per_launch_s * 1e3:.3f} ms",
         },
         "cpu_baseline": cpu,
         "plain_per_scene": {"value": plain["pu_per_s"], "scenes": len(sample),

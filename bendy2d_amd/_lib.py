@@ -74,7 +74,7 @@ class BatchConfig(Structure):
 
 class ServerConfig(Structure):
     _fields_ = [("memory_budget", c_uint64), ("steps_per_launch", c_uint32), ("max_active", c_uint32),
-                ("launch", BatchConfig)]
+                ("launch", BatchConfig), ("launch_budget_us", c_uint32), ("reserved", c_uint32)]
 
 
 class RequestDone(Structure):

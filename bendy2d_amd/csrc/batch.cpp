@@ -138,11 +138,14 @@ int BatchCore::enqueue_launch(uint32_t steps, std::string &err) {
 
 // Run `launches` launches of `steps` updates each; replays a captured graph
 // when configured.
-int BatchCore::run(uint32_t launches, uint32_t steps, std::string &err) {
+int BatchCore::run(uint32_t launches, uint32_t steps, std::string &err, bool open_timing, bool close_timing) {
     if (!launches) return BD_OK;
     int rc = upload_tables(err);
     if (rc) return rc;
-    BD_HIP(hipEventRecord(ev0, stream));
+    if (open_timing) {
+        BD_HIP(hipEventRecord(ev0, stream));
+        last_launches = 0;
+    }
     if (cfg.use_graph) {
         if (!graph_exec || graph_steps != steps) {
             invalidate_graph();
@@ -166,9 +169,11 @@ int BatchCore::run(uint32_t launches, uint32_t steps, std::string &err) {
             if (rc) return rc;
         }
     }
-    BD_HIP(hipEventRecord(ev1, stream));
-    timing_pending = true;
-    last_launches = launches * (1 + (n_small ? 1 : 0) + (n_large ? 1 : 0));
+    last_launches += launches * (1 + (n_small ? 1 : 0) + (n_large ? 1 : 0));
+    if (close_timing) {
+        BD_HIP(hipEventRecord(ev1, stream));
+        timing_pending = true;
+    }
     return BD_OK;
 }
 
@@ -310,9 +315,10 @@ int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps) {
         b->last_dt = dt;
     }
     uint32_t spl = std::max<uint32_t>(c.cfg.steps_per_launch, 1);
+    const uint32_t full = (steps - (any_acc ? 1u : 0u)) / spl, rem = (steps - (any_acc ? 1u : 0u)) % spl;
     if (any_acc) {
         // the first launch consumes the initial acc; the rest use plain descs
-        rc = c.run(1, 1, err);
+        rc = c.run(1, 1, err, true, !full && !rem);
         if (rc) return fail(rc, err);
         for (size_t i = 0; i < b->images.size(); i++) {
             c.descs[i].flags &= ~kFlagAcc;
@@ -321,11 +327,8 @@ int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps) {
         c.descs_dirty = true;
         steps -= 1;
     }
-    if (steps) {
-        uint32_t full = steps / spl, rem = steps % spl;
-        if (full) { rc = c.run(full, spl, err); if (rc) return fail(rc, err); }
-        if (rem) { rc = c.run(1, rem, err); if (rc) return fail(rc, err); }
-    }
+    if (full) { rc = c.run(full, spl, err, !any_acc, !rem); if (rc) return fail(rc, err); }
+    if (rem) { rc = c.run(1, rem, err, !any_acc && !full, true); if (rc) return fail(rc, err); }
     for (auto &u : b->updates_run) u += steps + (any_acc ? 1 : 0);
     return BD_OK;
 }

@@ -42,7 +42,10 @@ struct BatchCore {
     void build_order(const std::vector<uint32_t> &active);
     int upload_tables(std::string &err);
     int enqueue_launch(uint32_t steps, std::string &err);
-    int run(uint32_t launches, uint32_t steps, std::string &err);
+    // `launches` launches of `steps` updates each.  The device time between
+    // the first call with open_timing and the last with close_timing becomes
+    // last_ms (one update may take several run calls).
+    int run(uint32_t launches, uint32_t steps, std::string &err, bool open_timing = true, bool close_timing = true);
     int finish_timing(std::string &err);
     SceneState *states_dev() { return states_ptr; }
 };

@@ -163,14 +163,16 @@ struct CCWork {
 
 constexpr uint32_t kOutlierBit = 0x80000000u;
 
-constexpr uint32_t kOutCap = 256;   // circles with an outlying skin, per list build
+constexpr uint32_t kOutCap = 512;   // circles with an outlying skin, per list build
 constexpr uint32_t kMidList = 256;  // longer candidate lists are sorted by the whole workgroup
+constexpr uint32_t kNone = 0xffffffffu;
 
 struct CCShared {
     uint16_t out[kOutCap];   // outlier circles of the current list
     uint32_t hist[33];       // skins by log2(skin / base)
     uint32_t nout;
     uint32_t nmid, nlong;    // candidate lists sorted by a wavefront / by the workgroup
+    uint32_t halt;           // the scene's time budget for this launch is spent
     float red3[3 * 16];      // workgroup sums
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
 };
@@ -459,8 +461,8 @@ struct CCStats {
 // Returns the new error state (0 = fine).
 template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
-                                 const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, float &sbase,
-                                 bool &list_valid, CCStats &cs) {
+                                 const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, uint32_t *s_wl,
+                                 float &sbase, bool &list_valid, CCStats &cs) {
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
     for (int attempt = 0;; attempt++) {
@@ -479,7 +481,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             if (acc.z > 0.0f) u = make_float2(acc.x / acc.z, acc.y / acc.z);
             float bad = 0.0f;
             for (uint32_t c = tid; c < Nc; c += WG)
-                if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.4f * w.skin[c]) bad = 1.0f;
+                if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.6f * w.skin[c]) bad = 1.0f;
             rebuild = wg_max<WG>(bad, s_red) > 0.0f;
         }
         if (rebuild) {
@@ -632,30 +634,44 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             cs.t_lists += tb2 - tb1;
             cs.t_sort += tb3 - tb2;
         }
-        // ---- sweep: worklist rounds
+        // ---- sweep: worklist rounds.  nxp (in key, free outside list builds):
+        // each circle's next unprocessed partner, kNone when its list is done,
+        // so a pair (x, y) is ready exactly when nxp[x] == y and nxp[y] == x
+        uint32_t *nxp = w.key;
+        // worklists: the first WG ready pairs of a round in LDS (s_wl, two
+        // buffers of WG), the rest in HBM at the same index
+        uint32_t *const gwl[2] = {w.wl, w.wl + (Nc / 2 + 1)};
+        auto wl_put = [&](uint32_t b, uint32_t i, uint32_t v) {
+            if (i < (uint32_t)WG) s_wl[b * WG + i] = v; else gwl[b][i] = v;
+        };
+        auto wl_get = [&](uint32_t b, uint32_t i) { return i < (uint32_t)WG ? s_wl[b * WG + i] : gwl[b][i]; };
         for (uint32_t c = tid; c < Nc; c += WG) {
             w.save[c] = P[Np + c];
             w.path[c] = 0.0f;
             w.head[c] = 0u;
-            w.stamp[c] = 0xffffffffu;
+            w.stamp[c] = kNone;
+            nxp[c] = w.cnt[c] ? (uint32_t)w.cand[w.off[c]] : kNone;
         }
         if (tid == 0) { sh->cnt3[0] = 0u; sh->cnt3[1] = 0u; sh->cnt3[2] = 0u; }
         __syncthreads();
-        uint32_t *wl0 = w.wl, *wl1 = w.wl + (Nc / 2 + 1);
         for (uint32_t c = tid; c < Nc; c += WG) {
-            if (w.cnt[c] == 0) continue;
-            uint32_t o = w.cand[w.off[c]];
-            if (c < o && w.cand[w.off[o]] == c) wl0[atomicAdd(&sh->cnt3[0], 1u)] = c | (o << 16);
+            const uint32_t o = nxp[c];
+            if (o != kNone && c < o && nxp[o] == c) wl_put(0, atomicAdd(&sh->cnt3[0], 1u), c | (o << 16));
         }
         __syncthreads();
         uint32_t n = uni(sh->cnt3[0]);
         for (uint32_t r = 0; n > 0; r++) {
-            uint32_t *cur = (r & 1) ? wl1 : wl0, *nxt = (r & 1) ? wl0 : wl1;
-            // apply the ready pairs and move both heads
+            const uint32_t cb = r & 1u, nb = cb ^ 1u;
+            // apply the ready pairs and move both heads; a thread keeps its first
+            // pair and the two new next partners for the readiness check
+            uint32_t mc = 0, mo = 0, mnc = kNone, mno = kNone;
             for (uint32_t k = tid; k < n; k += WG) {
-                uint32_t pr = cur[k], c = pr & 0xffffu, o = pr >> 16;
+                const uint32_t pr = wl_get(cb, k), c = pr & 0xffffu, o = pr >> 16;
                 float2 pi = P[Np + c], pj = P[Np + o];
                 const uint32_t hc = w.head[c], ho = w.head[o];
+                const uint32_t oc = w.off[c], oo = w.off[o];
+                const uint32_t xc = hc + 1u < w.cnt[c] ? (uint32_t)w.cand[oc + hc + 1u] : kNone;
+                const uint32_t xo = ho + 1u < w.cnt[o] ? (uint32_t)w.cand[oo + ho + 1u] : kNone;
                 float qi = 0.0f, qj = 0.0f;
                 if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, crad[c], crad[o], qi, qj)) {
                     P[Np + c] = pi;
@@ -663,28 +679,32 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                     w.path[c] = excursion(w.path[c], pi, w.save[c]);
                     w.path[o] = excursion(w.path[o], pj, w.save[o]);
                 }
-                w.traj[w.off[c] + hc] = pi;
-                w.traj[w.off[o] + ho] = pj;
+                w.traj[oc + hc] = pi;
+                w.traj[oo + ho] = pj;
                 w.head[c] = hc + 1u;
                 w.head[o] = ho + 1u;
+                nxp[c] = xc;
+                nxp[o] = xo;
                 w.stamp[c] = r;
                 w.stamp[o] = r;
+                if (k == (uint32_t)tid) { mc = c; mo = o; mnc = xc; mno = xo; }
             }
             if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;  // read at the end of round r-1, two barriers ago
             __syncthreads();
             // pairs completed by a moved head
             for (uint32_t k = tid; k < n; k += WG) {
-                uint32_t pr = cur[k];
+                uint32_t c = mc, o = mo, nc = mnc, no = mno;
+                if (k != (uint32_t)tid) {
+                    const uint32_t pr = wl_get(cb, k);
+                    c = pr & 0xffffu; o = pr >> 16;
+                    nc = nxp[c]; no = nxp[o];
+                }
                 for (int e = 0; e < 2; e++) {
-                    uint32_t x = e ? (pr >> 16) : (pr & 0xffffu);
-                    uint32_t hx = w.head[x];
-                    if (hx >= w.cnt[x]) continue;
-                    uint32_t y = w.cand[w.off[x] + hx];
-                    uint32_t hy = w.head[y];
-                    if (hy >= w.cnt[y] || w.cand[w.off[y] + hy] != x) continue;
+                    const uint32_t x = e ? o : c, y = e ? no : nc;
+                    if (y == kNone || nxp[y] != x) continue;
                     if (w.stamp[y] == r && y < x) continue;  // both moved: the smaller one adds it
-                    uint32_t a = x < y ? x : y, b = x < y ? y : x;
-                    nxt[atomicAdd(&sh->cnt3[(r + 1) % 3], 1u)] = a | (b << 16);
+                    const uint32_t a = x < y ? x : y, b = x < y ? y : x;
+                    wl_put(nb, atomicAdd(&sh->cnt3[(r + 1) % 3], 1u), a | (b << 16));
                 }
             }
             __syncthreads();
@@ -704,7 +724,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             float pth = w.path[c];
             float v = moved_bound(w.save[c], w.ref[c], u, pth);
             // the sweep's own pushes, decaying slowly so circles in lasting contact keep their skin
-            w.need[c] = (pth <= 3.0e38f) ? fmaxf(2.5f * pth, 0.75f * w.need[c]) : 0.0f;
+            w.need[c] = (pth <= 3.0e38f) ? fmaxf(1.5f * pth, 0.75f * w.need[c]) : 0.0f;
             if (v <= 0.8f * w.skin[c]) {
                 vmax = fmaxf(vmax, v);
                 vb[c] = v;
@@ -735,7 +755,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
 template <int WG, int MODE>
 __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint32_t steps, SceneState *st, float2 *s_lds,
                           ColliderStage *s_col,
-                          float *s_red, uint32_t *s_u32, CCShared *s_cc) {
+                          float *s_red, uint32_t *s_u32, CCShared *s_cc, uint32_t *s_wl) {
     // fields are read where used: the descriptor is uniform and read-only, so
     // they come through scalar loads instead of a private copy
     const SceneDesc &d = *dp;
@@ -799,6 +819,8 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     bool acc = (d.flags & kFlagAcc) != 0;
     const float2 *acc0 = (const float2 *)(base + d.acc0_off);
 
+    const uint64_t t_scene0 = wall_clock64();
+    uint32_t ran = 0;
     for (uint32_t step = 0; step < steps; step++) {
         for (uint32_t sub = 0; sub < d.sub_steps; sub++) {
             // ---- links: dependency layers, k sweeps
@@ -827,7 +849,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             if (Nc >= 2 && !err)
             {
                 const uint64_t t0 = wall_clock64();
-                err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, skin, list_valid, cs);
+                err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl, skin, list_valid, cs);
                 t_sweep += wall_clock64() - t0;
             }
             // ---- bodies: colliders -> bounds -> Verlet
@@ -858,6 +880,14 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             acc = false;
             __syncthreads();
             t_bodies += wall_clock64() - tq0;
+        }
+        ran = step + 1;
+        if (d.budget && ran < steps) {
+            // stop after a whole update once the budget is spent (one clock read,
+            // shared, so the whole workgroup leaves together)
+            if (tid == 0) s_cc->halt = (wall_clock64() - t_scene0 > (uint64_t)d.budget) ? 1u : 0u;
+            __syncthreads();
+            if (uni(s_cc->halt)) break;
         }
     }
     // write back
@@ -892,6 +922,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         st->t_lists += cs.t_lists;
         st->t_sort += cs.t_sort;
         st->kept += cs.kept;
+        st->last_steps = ran;
     }
     __syncthreads();
 }
@@ -905,6 +936,7 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
     __shared__ float s_red[WG / 64];
     __shared__ uint32_t s_u32[WG / 64];
     __shared__ CCShared s_cc;
+    __shared__ uint32_t s_wl[2 * WG];
     __shared__ uint32_t s_next;
     for (;;) {
         if (threadIdx.x == 0) s_next = atomicAdd(queue, 1u);
@@ -917,11 +949,11 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
         const uint32_t steps = d->steps < launch_steps ? d->steps : launch_steps;
         if (steps == 0 || d->N == 0) continue;
         if (16ull * d->N <= lds_bytes)
-            run_scene<WG, 2>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
+            run_scene<WG, 2>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl);
         else if (8ull * d->N <= lds_bytes)
-            run_scene<WG, 1>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
+            run_scene<WG, 1>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl);
         else
-            run_scene<WG, 0>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc);
+            run_scene<WG, 0>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl);
     }
 }
 

@@ -42,7 +42,10 @@ struct SceneDesc {
     float lox, loy, hix, hiy;
     float rmax;
     uint32_t state_slot;         // index of this scene's SceneState
-    uint32_t pad[2];
+    uint32_t budget;             // device wall-clock ticks after which the scene stops at the
+                                 // end of an update (0: run all `steps`); SceneState::last_steps
+                                 // tells how many ran
+    uint32_t pad;
 };
 
 constexpr uint32_t kFlagPins = 1u;        // some particle is pinned
@@ -67,6 +70,7 @@ struct SceneState {
     float s_cell, smax;
     uint64_t t_count, t_lists, t_sort;  // list build split: counts, lists, sorting (ticks)
     uint32_t kept;  // sweeps kept although circles left their skins (no missed pair touched)
+    uint32_t last_steps;  // updates the scene's last launch ran
 };
 static_assert(sizeof(SceneState) <= 128, "SceneState must fit the single-scene device blob");
 

@@ -121,6 +121,8 @@ struct bd_server {
     // statistics (scheduler thread; read under mu)
     uint64_t launches = 0, pu = 0, n_done = 0, evictions = 0, readmissions = 0, waiting = 0;
     double busy_s = 0.0;
+    uint32_t budget_ticks = 0;           // per-scene launch budget in device wall-clock ticks
+    std::vector<SceneState> readback;    // states of the last launch's slot range
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -235,6 +237,7 @@ static void scheduler(bd_server *S) {
             SceneDesc d = sc->desc;
             d.base = (uint64_t)(uintptr_t)(S->arena.base + sc->off);
             d.state_slot = sc->slot;
+            d.budget = S->budget_ticks;
             set_scene_params(d, sp, std::min(rq.remaining, spl), sc->acc_pending);
             c.descs[i] = d;
         }
@@ -245,12 +248,28 @@ static void scheduler(bd_server *S) {
         int rc = c.run(1, spl, err);
         if (rc == BD_OK) rc = c.finish_timing(err);
         if (rc != BD_OK) { S->fatal = err; return; }
+        // with a time budget, each scene reports the updates it ran
+        uint32_t lo_slot = 0;
+        if (S->budget_ticks) {
+            uint32_t hi_slot = 0;
+            lo_slot = UINT32_MAX;
+            for (ServedScene *sc : launch) { lo_slot = std::min(lo_slot, sc->slot); hi_slot = std::max(hi_slot, sc->slot); }
+            S->readback.resize(hi_slot - lo_slot + 1);
+            if (hipMemcpyAsync(S->readback.data(), S->slots.p + lo_slot, S->readback.size() * sizeof(SceneState),
+                               hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+                hipStreamSynchronize(c.stream) != hipSuccess) {
+                S->fatal = "scene state readback failed";
+                return;
+            }
+        }
         const double t_done = S->now();
         std::vector<bd_request_done> fin;
         uint64_t pu = 0;
         for (ServedScene *sc : launch) {
             Request &rq = sc->queue.front();
             uint32_t ran = std::min(rq.remaining, spl);
+            if (S->budget_ticks && sc->desc.N)
+                ran = std::max(1u, std::min(ran, S->readback[sc->slot - lo_slot].last_steps));
             rq.remaining -= ran;
             sc->updates_run += ran;
             sc->acc_pending = false;
@@ -294,6 +313,16 @@ bd_server *bd_server_new(const bd_server_config *cfg) {
     }
     std::string err;
     if (S->core.init_device(err)) { set_last_error(err); return nullptr; }
+    if (cfg->launch_budget_us) {
+        int dev = 0, khz = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+            set_last_error("device wall clock rate unavailable");
+            return nullptr;
+        }
+        const uint64_t ticks = (uint64_t)cfg->launch_budget_us * (uint64_t)khz / 1000u;
+        S->budget_ticks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ticks, 1), UINT32_MAX);
+    }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { set_last_error("hipMemGetInfo failed"); return nullptr; }
     uint64_t budget = std::min<uint64_t>(cfg->memory_budget, (uint64_t)(free_b * 0.9));

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (BD_ENGINE_AUTO, BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BatchConfig, Bounds,
+from ._lib import (BD_ENGINE_AUTO, BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BD_ERR_STATE, BatchConfig, Bounds,
                    Circle, Link, Particle, PolygonIn, PolygonInfo, RequestDone, ServerConfig, ServerStats, Stats,
                    Vec2, check, lib)
 
@@ -374,11 +374,12 @@ class Batch:
 class Server:
     """Continuous-batching server of scene-advance requests (bd_server_*)."""
 
-    def __init__(self, memory_budget=64 << 30, steps_per_launch=8, max_active=0, launch=None):
+    def __init__(self, memory_budget=64 << 30, steps_per_launch=8, max_active=0, launch=None, launch_budget_us=0):
         cfg = ServerConfig()
         cfg.memory_budget = int(memory_budget)
         cfg.steps_per_launch = int(steps_per_launch)
         cfg.max_active = int(max_active)
+        cfg.launch_budget_us = int(launch_budget_us)
         if launch:
             for k, v in launch.items():
                 setattr(cfg.launch, k, int(v))
@@ -420,7 +421,12 @@ class Server:
                 return out
 
     def wait_idle(self, timeout=60.0):
-        check(lib().bd_server_wait_idle(self._h, float(timeout)), "server_wait_idle")
+        """True once no request is pending; False if `timeout` seconds pass first."""
+        rc = lib().bd_server_wait_idle(self._h, float(timeout))
+        if rc == BD_ERR_STATE:
+            return False
+        check(rc, "server_wait_idle")
+        return True
 
     def read_scene(self, scene_id, solver):
         check(lib().bd_server_read_scene(self._h, int(scene_id), solver._h), "server_read_scene")

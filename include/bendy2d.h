@@ -348,6 +348,9 @@ typedef struct bd_server_config {
     uint32_t steps_per_launch;  /* updates per launch for each active scene */
     uint32_t max_active;        /* scenes per launch at most (0 = no limit) */
     bd_batch_config launch;     /* launch shape (wg sizes, split, LDS); steps_per_launch and use_graph ignored */
+    uint32_t launch_budget_us;  /* a scene stops after the update during which it passed this much device
+                                   time in a launch, so slow scenes do not hold up the rest (0: no limit) */
+    uint32_t reserved;          /* zero */
 } bd_server_config;
 
 /* A finished request.  Times are seconds on the server's monotonic clock

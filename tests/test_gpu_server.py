@@ -26,9 +26,12 @@ def oracle_state(s, steps_dts):
     return state(o)
 
 
-def test_mix_matches_oracle():
+@pytest.mark.parametrize("budget_us", [0, 1])
+def test_mix_matches_oracle(budget_us):
+    # budget_us=1: every scene stops after one update per launch, so requests
+    # finish over several launches in uneven steps
     rng = np.random.default_rng(3)
-    srv = Server(memory_budget=1 << 30, steps_per_launch=4)
+    srv = Server(memory_budget=1 << 30, steps_per_launch=4, launch_budget_us=budget_us)
     scenes, ids, history = [], [], []
     for i in range(12):
         s = scene(200 + i)
@@ -42,7 +45,7 @@ def test_mix_matches_oracle():
         dt = DT if j % 3 else 1.0 / 120.0
         srv.advance(ids[i], k, dt)
         history[i].append((k, dt))
-    srv.wait_idle(120)
+    assert srv.wait_idle(120)
     done = srv.poll()
     assert len(done) == 32 and all(d["status"] == 0 for d in done)
     assert all(d["t_done"] >= d["t_start"] >= d["t_submit"] for d in done)
@@ -53,6 +56,10 @@ def test_mix_matches_oracle():
         assert bits_equal(got, want), f"scene {i}: {first_diff(got, want)}"
     st = srv.stats()
     assert st["requests_done"] == 32 and st["requests_pending"] == 0
+    if budget_us:
+        # one update per scene and launch: at least as many launches as the
+        # longest scene history has updates
+        assert st["launches"] >= max(sum(k for k, _ in h) for h in history)
     srv.close()
 
 
@@ -62,7 +69,7 @@ def test_eviction_and_readmission():
     s0 = scene(300, max_side=8)
     one = Server(memory_budget=1 << 30)
     one.create(s0.clone(), 1, DT)
-    one.wait_idle(60)
+    assert one.wait_idle(60)
     chunk = one.stats()["bytes_used"]
     one.close()
     srv = Server(memory_budget=3 * chunk + 1024, steps_per_launch=2)
@@ -70,10 +77,10 @@ def test_eviction_and_readmission():
     ids = []
     for s in scenes:
         ids.append(srv.create(s, 3, DT)[1])
-        srv.wait_idle(60)
+        assert srv.wait_idle(60)
     for sid in ids:
         srv.advance(sid, 2, DT)
-        srv.wait_idle(60)
+        assert srv.wait_idle(60)
     st = srv.stats()
     assert st["evictions"] > 0 and st["readmissions"] > 0
     for i, s in enumerate(scenes):
@@ -87,7 +94,7 @@ def test_alone_equals_in_mix():
     target = scene(777, max_side=24)
     alone = Server(memory_budget=1 << 30, steps_per_launch=3)
     sid = alone.create(target.clone(), 10, DT)[1]
-    alone.wait_idle(60)
+    assert alone.wait_idle(60)
     a = target.clone()
     alone.read_scene(sid, a)
     alone.close()
@@ -98,7 +105,7 @@ def test_alone_equals_in_mix():
     sid2 = mix.create(target.clone(), 10, DT)[1]
     for o in others[15:]:
         mix.create(o, 13, DT)
-    mix.wait_idle(60)
+    assert mix.wait_idle(60)
     b = target.clone()
     mix.read_scene(sid2, b)
     mix.close()
