@@ -204,16 +204,17 @@ def bench_serving(seconds, rate, launch_cfg, steps_per_launch=8, budget=64 << 30
 
 
 def copy_calibration(nbytes=1 << 30):
-    """One plain device copy of nbytes (torch), for scaling the HBM byte
-    counters of a profiler run: its FETCH_SIZE and WRITE_SIZE should each read
-    nbytes."""
-    import torch
-    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    y.copy_(x)
-    torch.cuda.synchronize()
-    return nbytes
+    """One plain device copy of nbytes by the library's streaming copy kernel
+    (k_copy_f4), for scaling the HBM byte counters of a profiler run (its
+    FETCH_SIZE and WRITE_SIZE should each read nbytes) and for the achievable
+    copy bandwidth (read + write bytes over the kernel's time)."""
+    import ctypes
+    from bendy2d_amd import lib
+    ms = ctypes.c_float()
+    rc = lib().bd_ext_copy_calibration(nbytes, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"bd_ext_copy_calibration failed: {rc}")
+    return {"bytes": nbytes, "ms": ms.value, "copy_GBps": 2.0 * nbytes / (ms.value * 1e-3) / 1e9}
 
 
 def cpu_baseline(seconds=12.0, seed0=1000):
@@ -237,6 +238,19 @@ def cpu_baseline(seconds=12.0, seed0=1000):
     return {"value": pu / busy, "unit": "particle-updates/s", "cores": 1, "kind": "port",
             "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off), 1 thread: the first {n} scenes of the "
                       f"4096-scene workload, 1 update (8 substeps) each, {busy:.1f} s CPU"}
+
+
+def profiled_traffic_per_update():
+    """HBM bytes per update of the 4,096-scene batch, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel);
+    None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_scenes_batched_softbody4096.md")
+    try:
+        text = open(path).read()
+        block = text.rsplit("```json", 1)[1].split("```", 1)[0]
+        return float(json.loads(block)["traffic_bytes_per_update"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
 
 
 def cpu_model():
@@ -278,7 +292,7 @@ def main():
     launch = json.loads(args.launch) if args.launch else None
 
     if args.only == "copy":
-        log(f"copy calibration: {copy_calibration()} bytes")
+        log(f"copy calibration: {copy_calibration()}")
         return
     if args.only == "cloth":
         log(f"cloth {args.cloth}^2: {bench_cloth(args.cloth, args.steps, args.warmup)}")
@@ -290,8 +304,9 @@ def main():
     bat = bench_batched(scenes, pts, links, args.steps, args.warmup, tune=not args.no_tune, launch=launch)
     log(f"batched: {bat['pu_per_s']:.3e} PU/s, {bat['wall_s'] * 1e3 / bat['steps']:.3f} ms/step, "
         f"device {bat['device_ms']:.1f} ms over {bat['replays']} launches, cc {bat['cc']}")
+    copy = copy_calibration()
+    log(f"copy calibration: {copy}")
     if args.only == "batched":
-        log(f"copy calibration: {copy_calibration()} bytes")
         return
 
     sample = scenes if not args.plain_sample else scenes[::max(1, len(scenes) // args.plain_sample)]
@@ -324,6 +339,9 @@ def main():
     per_launch_s = bat["device_ms"] / 1e3 / max(bat["replays"], 1)
     bytes_per_launch = bat["alg_bytes"] / max(bat["replays"], 1)
     achieved = bytes_per_launch / per_launch_s
+    spl = int(bat["config"]["steps_per_launch"])
+    tpu = profiled_traffic_per_update() if args.scenes == 4096 else None
+    traffic = tpu * spl if tpu else None
     out = {
         "metric": "particle-updates/s batched vs plain (1 GPU) + p50/p99 request latency",
         "value": bat["pu_per_s"],
@@ -347,7 +365,10 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None,
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "traffic_source": "profiles/k_scenes_batched_softbody4096.md: rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                              "update, scaled by a 1 GiB copy kernel, times this run's updates per launch",
+            "measured_copy_GBps": copy["copy_GBps"],
             "kernel": "k_scenes, one launch = both size classes concurrently, "
                       f"{bat['config']['steps_per_launch']} updates; {bytes_per_launch / 1e9:.3f} GB algorithmic "
                       f"per launch in {per_launch_s * 1e3:.3f} ms",

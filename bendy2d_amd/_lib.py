@@ -179,6 +179,7 @@ _SIGS = [
     ("bd_batch_last_timing", c_int32, [_S, POINTER(c_float), POINTER(c_uint32)]),
     ("bd_batch_cc_stats", c_int32, [_S, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     ("bd_batch_scene_times", c_int32, [_S, c_int64] + [POINTER(ctypes.c_double)] * 4),
+    ("bd_ext_copy_calibration", c_int32, [c_uint64, POINTER(c_float)]),
     ("bd_batch_scene_lists", c_int32, [_S, c_int64, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
                                        POINTER(c_float), POINTER(c_float)] + [POINTER(ctypes.c_double)] * 3
                                       + [POINTER(c_uint32)]),

@@ -1,6 +1,8 @@
 // util.hip -- fill and device-wide exclusive scan.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "../../include/bendy2d.h"
 #include "kernels.h"
 
@@ -16,6 +18,13 @@ void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream) {
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+}
+
+// Plain streaming copy, 16 bytes per lane and iteration: the reference kernel
+// for reading HBM byte counters and for the achievable copy bandwidth.
+__global__ void k_copy_f4(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
 constexpr int kScanThreads = 256;
@@ -96,3 +105,33 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, hipStream_t 
 }
 
 }  // namespace bd
+
+extern "C" int32_t bd_ext_copy_calibration(uint64_t bytes, float *ms) {
+    if (bytes < 16 || (bytes & 15)) return BD_ERR_ARG;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev < 1) return BD_ERR_NODEVICE;
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int32_t rc = BD_OK;
+    if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) rc = BD_ERR_NOMEM;
+    if (rc == BD_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = BD_ERR_DEVICE;
+    if (rc == BD_OK) {
+        const size_t n = bytes / 16;
+        bd::launch_fill_u32((uint32_t *)src, bytes / 4, 0x3f800000u, nullptr);
+        // one float4 per lane (the grid-stride loop only matters past 2^28 bytes)
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)1 << 20);
+        float t = 0.0f;
+        if (hipEventRecord(e0, nullptr) != hipSuccess) rc = BD_ERR_DEVICE;
+        if (rc == BD_OK) hipLaunchKernelGGL(bd::k_copy_f4, dim3(blocks), dim3(256), 0, nullptr,
+                                            (const float4 *)src, (float4 *)dst, n);
+        if (rc == BD_OK && (hipGetLastError() != hipSuccess || hipEventRecord(e1, nullptr) != hipSuccess ||
+                            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = BD_ERR_DEVICE;
+        if (ms) *ms = t;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return rc;
+}

@@ -442,6 +442,12 @@ int64_t bd_scene_softbody(bd_solver *s, uint64_t seed, uint32_t max_side);
 int32_t bd_device_count(void);
 /* Library build identification string. */
 const char *bd_version(void);
+/* Measurement helper, no counterpart in the reference: one plain device copy
+ * of `bytes` (a multiple of 16) between two fresh buffers by a streaming
+ * kernel, timed with HIP events (*ms).  Profiler runs read the HBM byte
+ * counters of this kernel to scale those of the solver kernels, and bytes x 2
+ * / time is the achievable copy bandwidth. */
+int32_t bd_ext_copy_calibration(uint64_t bytes, float *ms);
 
 #ifdef __cplusplus
 }
