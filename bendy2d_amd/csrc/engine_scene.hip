@@ -472,6 +472,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         if (!rebuild) {
             // drift since the list was built, relative to the mean shift u
             float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll 4
             for (uint32_t c = tid; c < Nc; c += WG) {
                 float2 p = P[Np + c], r0 = w.ref[c];
                 float dx = p.x - r0.x, dy = p.y - r0.y;
@@ -480,9 +481,16 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             acc = wg_sum3<WG>(acc, sh->red3);
             if (acc.z > 0.0f) u = make_float2(acc.x / acc.z, acc.y / acc.z);
             float bad = 0.0f;
+#pragma unroll 4
             for (uint32_t c = tid; c < Nc; c += WG)
                 if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.6f * w.skin[c]) bad = 1.0f;
             rebuild = wg_max<WG>(bad, s_red) > 0.0f;
+            // no candidate pair at all and every circle within its skin: no pair
+            // can overlap during the sweep, which therefore changes nothing
+            if (!rebuild && cs.pairs == 0u) {
+                cs.since_build++;
+                return 0u;
+            }
         }
         if (rebuild) {
             cs.rebuilds++;
@@ -828,6 +836,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             for (uint32_t it = 0; it < d.link_iters; it++) {
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = loff[L], hi = loff[L + 1];
+#pragma unroll 4
                     for (uint32_t l = lo + tid; l < hi; l += WG) {
                         LinkS rec = links[l];
                         uint32_t a = rec.a & 0x7fffu, b = rec.b & 0x7fffu;
