@@ -165,6 +165,8 @@ constexpr uint32_t kOutlierBit = 0x80000000u;
 
 constexpr uint32_t kOutCap = 512;   // circles with an outlying skin, per list build
 constexpr uint32_t kMidList = 256;  // longer candidate lists are sorted by the whole workgroup
+constexpr uint32_t kSlot = 24;      // normal partners kept from the counting pass (more are gathered again)
+static_assert(2 * kSlot <= 8 * kCandCap, "count-pass slots must fit in the trajectory array");
 constexpr uint32_t kNone = 0xffffffffu;
 
 struct CCShared {
@@ -185,9 +187,10 @@ struct CCShared {
 // still a true partner when it passes the distance test, and it is in no
 // other scanned bucket).  An outlier (larger skin) finds all its partners
 // itself and writes itself into its normal partners' lists too, so normal
-// circles never search far.  out == nullptr counts only.
-__device__ __forceinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, float2 p, float ec, double inv_h,
-                                                  uint32_t tmask, uint16_t *out) {
+// circles never search far.  The first `cap` partners go to out; the count
+// is returned in full.
+__device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, float2 p, float ec, double inv_h,
+                                                  uint32_t tmask, uint16_t *out, uint32_t cap) {
     const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
     uint32_t kk[9];
     bool dup[9];
@@ -212,7 +215,7 @@ __device__ __forceinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, f
             float ex = p.x - r.x, ey = p.y - r.y;
             float lim = ec + r.z;
             if (ex * ex + ey * ey < lim * lim) {
-                if (out) out[n] = (uint16_t)o;
+                if (n < cap) out[n] = (uint16_t)o;
                 n++;
             }
         }
@@ -546,11 +549,15 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                     make_float4(p.x, p.y, crad[c] + sk, __uint_as_float(c | (sk > s_cell ? kOutlierBit : 0u)));
             }
             __syncthreads();
-            // counts: normal circles their normal partners (kept in stamp until the
-            // sweep needs it), then outliers add themselves to their partners
+            // counts: normal circles gather their normal partners into a slot of
+            // kSlot entries (in traj, free until the sweep) and keep the count in
+            // stamp; then outliers add themselves to their partners' counts
+            uint16_t *const slot = (uint16_t *)w.traj;
             for (uint32_t c = tid; c < Nc; c += WG) {
                 const float sk = w.skin[c];
-                uint32_t n = sk > s_cell ? 0u : gather_normal(c, w, w.ref[c], crad[c] + sk, inv_h, tmask, nullptr);
+                uint32_t n = sk > s_cell ? 0u
+                                         : gather_normal(c, w, w.ref[c], crad[c] + sk, inv_h, tmask,
+                                                         slot + (size_t)c * kSlot, kSlot);
                 w.cnt[c] = n;
                 w.stamp[c] = n;
             }
@@ -574,11 +581,21 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 if (sbase < 1e-4f * fmaxf(rmax, 1e-3f)) return 1u;
                 continue;
             }
-            // lists: normal partners first, outliers append at a cursor (in head)
+            // lists: normal partners first (from the slot, or gathered again when
+            // they did not fit it), outliers append at a cursor (in head)
             for (uint32_t c = tid; c < Nc; c += WG) {
                 const float sk = w.skin[c];
                 uint32_t own = 0;
-                if (sk <= s_cell) own = gather_normal(c, w, w.ref[c], crad[c] + sk, inv_h, tmask, w.cand + w.off[c]);
+                if (sk <= s_cell) {
+                    own = w.stamp[c];
+                    uint16_t *dst = w.cand + w.off[c];
+                    if (own <= kSlot) {
+                        const uint16_t *src = slot + (size_t)c * kSlot;
+                        for (uint32_t i = 0; i < own; i++) dst[i] = src[i];
+                    } else {
+                        gather_normal(c, w, w.ref[c], crad[c] + sk, inv_h, tmask, dst, own);
+                    }
+                }
                 w.head[c] = w.off[c] + own;
             }
             __syncthreads();
