@@ -176,8 +176,58 @@ struct CCShared {
     uint32_t nmid, nlong;    // candidate lists sorted by a wavefront / by the workgroup
     uint32_t halt;           // the scene's time budget for this launch is spent
     float red3[3 * 16];      // workgroup sums
+    float red9[9 * 16];      // workgroup drift summary (see drift_summary)
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
 };
+
+// Drift of the circles from their list-build positions, in one pass: the sum
+// and count of the finite displacements, their bounding box, the smallest
+// skin, and whether some circle has a finite position but a non-finite
+// list-build one.  Every thread gets the workgroup's result.
+struct Drift {
+    float sx, sy, n, xmin, xmax, ymin, ymax, smin, nonfin;
+};
+
+template <int WG>
+__device__ __forceinline__ Drift drift_summary(const float2 *P, uint32_t Np, uint32_t Nc, const CCWork &w,
+                                               float *s_red9) {
+    Drift d{0.0f, 0.0f, 0.0f, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, 0.0f};
+#pragma unroll 2
+    for (uint32_t c = threadIdx.x; c < Nc; c += WG) {
+        const float2 p = P[Np + c], r0 = w.ref[c];
+        d.smin = fminf(d.smin, w.skin[c]);
+        if (!(fabsf(p.x) <= 3.0e38f) || !(fabsf(p.y) <= 3.0e38f)) continue;  // touches nothing
+        const float dx = p.x - r0.x, dy = p.y - r0.y;
+        if (fabsf(dx) <= 3.0e38f && fabsf(dy) <= 3.0e38f) {
+            d.sx += dx; d.sy += dy; d.n += 1.0f;
+            d.xmin = fminf(d.xmin, dx); d.xmax = fmaxf(d.xmax, dx);
+            d.ymin = fminf(d.ymin, dy); d.ymax = fmaxf(d.ymax, dy);
+        } else {
+            d.nonfin = 1.0f;
+        }
+    }
+    float v[9] = {d.sx, d.sy, d.n, d.xmin, d.xmax, d.ymin, d.ymax, d.smin, d.nonfin};
+    auto comb = [](int i, float a, float b) {
+        return i < 3 ? a + b : (i == 3 || i == 5 || i == 7) ? fminf(a, b) : fmaxf(a, b);
+    };
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int i = 0; i < 9; i++) v[i] = comb(i, v[i], __shfl_xor(v[i], o, 64));
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 9; i++) s_red9[9 * wave + i] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 9; i++) v[i] = s_red9[i];
+#pragma unroll 1
+    for (int wv = 1; wv < WG / 64; wv++)
+#pragma unroll
+        for (int i = 0; i < 9; i++) v[i] = comb(i, v[i], s_red9[9 * wv + i]);
+    return Drift{uni(v[0]), uni(v[1]), uni(v[2]), uni(v[3]), uni(v[4]), uni(v[5]), uni(v[6]), uni(v[7]), uni(v[8])};
+}
 
 // Candidate pairs are the pairs closer than (r_c + skin_c) + (r_o + skin_o)
 // when the list is built.  A normal circle (skin <= s_cell) finds its normal
@@ -473,21 +523,24 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         bool rebuild = !list_valid;
         float2 u = make_float2(0.0f, 0.0f);
         if (!rebuild) {
-            // drift since the list was built, relative to the mean shift u
-            float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+            // drift since the list was built, relative to the mean shift u.  When
+            // the displacements' bounding box lies within 0.6 of the smallest
+            // skin around u (with a margin for rounding), every circle does;
+            // otherwise each circle is checked
+            const Drift dr = drift_summary<WG>(P, Np, Nc, w, sh->red9);
+            if (dr.n > 0.0f) u = make_float2(dr.sx / dr.n, dr.sy / dr.n);
+            const float bx = fmaxf(fabsf(dr.xmin - u.x), fabsf(dr.xmax - u.x));
+            const float by = fmaxf(fabsf(dr.ymin - u.y), fabsf(dr.ymax - u.y));
+            const float box = dr.n > 0.0f ? sqrtf(bx * bx + by * by) : 0.0f;
+            if (dr.nonfin > 0.0f) {
+                rebuild = true;  // a circle came back from a non-finite list-build position
+            } else if (!(box * 1.0001f <= 0.6f * dr.smin * 0.9999f)) {
+                float bad = 0.0f;
 #pragma unroll 4
-            for (uint32_t c = tid; c < Nc; c += WG) {
-                float2 p = P[Np + c], r0 = w.ref[c];
-                float dx = p.x - r0.x, dy = p.y - r0.y;
-                if (fabsf(dx) <= 3.0e38f && fabsf(dy) <= 3.0e38f) { acc.x += dx; acc.y += dy; acc.z += 1.0f; }
+                for (uint32_t c = tid; c < Nc; c += WG)
+                    if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.6f * w.skin[c]) bad = 1.0f;
+                rebuild = wg_max<WG>(bad, s_red) > 0.0f;
             }
-            acc = wg_sum3<WG>(acc, sh->red3);
-            if (acc.z > 0.0f) u = make_float2(acc.x / acc.z, acc.y / acc.z);
-            float bad = 0.0f;
-#pragma unroll 4
-            for (uint32_t c = tid; c < Nc; c += WG)
-                if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.6f * w.skin[c]) bad = 1.0f;
-            rebuild = wg_max<WG>(bad, s_red) > 0.0f;
             // no candidate pair at all and every circle within its skin: no pair
             // can overlap during the sweep, which therefore changes nothing
             if (!rebuild && cs.pairs == 0u) {
