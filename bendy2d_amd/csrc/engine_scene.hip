@@ -1,19 +1,19 @@
 // engine_scene.hip -- SCENE engine: one workgroup runs a whole scene.
 //
 // A persistent kernel takes scenes from a work queue (largest first).  For
-// each scene the workgroup keeps positions in LDS (or in global memory for
-// scenes too large for the LDS budget), keeps each lane's previous positions
-// in registers, and runs all updates and substeps of the launch with
+// each scene the workgroup keeps positions and previous positions in LDS
+// when both fit the LDS budget, positions alone when only they fit, or works
+// from global memory; it runs all updates and substeps of the launch with
 // workgroup barriers between dependent phases (src/solver.rs:106-116):
 //
 //   links     dependency layers streamed from global memory, a barrier
 //             between layers (src/solver.rs:143-153)
 //   circles   exact replay of the (i, j) lexicographic pair sweep
-//             (src/solver.rs:168-177) from a Verlet candidate list: pairs
-//             within r_i + r_j + skin of each other when the list was built;
-//             a round runs every pair whose earlier pairs (sharing a circle)
-//             are done; a sweep whose circles moved 0.45 * skin or more since
-//             the list was built is undone and rerun with a new list
+//             (src/solver.rs:168-177) from per-circle candidate lists: a
+//             round runs every pair whose earlier pairs (sharing a circle)
+//             are done; a sweep stands when no pair missing from the lists
+//             can have touched at its turn, else it is undone and rerun with
+//             new lists (circle_sweep)
 //   bodies    EXT4 static colliders (staged in LDS), bounds, Verlet
 //             (src/solver.rs:155-165, 118-128)
 #include <hip/hip_runtime.h>
@@ -96,10 +96,10 @@ __device__ void wg_scan_global(uint32_t *a, uint32_t n, uint32_t *s_wave) {
 
 // Upper bound on how far a circle can have been from its list-build position
 // `r0` during a sweep that started at `p` and took it at most `path` away from
-// `p` (its largest excursion, see below).  A
-// circle at a non-finite position takes part in no pair (every comparison
-// with it is false), so it cannot invalidate the list; one whose list-build
-// position was non-finite but is finite now forces a rebuild.
+// `p` (its largest excursion, see below).  A circle at a non-finite position
+// takes part in no pair (every comparison with it is false), so it cannot
+// invalidate the list; one whose list-build position was non-finite but is
+// finite now forces a rebuild.
 //
 // The bound is taken relative to a common shift u of the whole scene: for a
 // pair, |(p_i - r_i) - (p_j - r_j)| <= |p_i - r_i - u| + |p_j - r_j - u| for
@@ -123,24 +123,6 @@ __device__ __forceinline__ float excursion(float cur, float2 p, float2 s) {
     return fmaxf(cur, d);
 }
 
-// Sum of (x, y, count) over the workgroup (every thread gets the result).
-template <int WG>
-__device__ __forceinline__ float3 wg_sum3(float3 v, float *s_red3) {
-    for (int o = 32; o > 0; o >>= 1) {
-        v.x += __shfl_xor(v.x, o, 64);
-        v.y += __shfl_xor(v.y, o, 64);
-        v.z += __shfl_xor(v.z, o, 64);
-    }
-    const int wave = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) { s_red3[3 * wave] = v.x; s_red3[3 * wave + 1] = v.y; s_red3[3 * wave + 2] = v.z; }
-    __syncthreads();
-    float3 t = make_float3(0.0f, 0.0f, 0.0f);
-#pragma unroll 2
-    for (int w = 0; w < WG / 64; w++) { t.x += s_red3[3 * w]; t.y += s_red3[3 * w + 1]; t.z += s_red3[3 * w + 2]; }
-    return make_float3(uni(t.x), uni(t.y), uni(t.z));
-}
-
 __device__ __forceinline__ int scell(float x, double inv_h) {
     double c = floor((double)x * inv_h);
     if (!(c > -1073741824.0)) c = -1073741824.0;
@@ -148,8 +130,7 @@ __device__ __forceinline__ int scell(float x, double inv_h) {
     return (int)c;
 }
 
-// MODE 2: pos and prev in LDS; MODE 1: pos in LDS, prev in global memory;
-// MODE 0: both in global memory (the scene chunk).
+// The circle-pair workspace of a scene, in its chunk (layout: cc_layout).
 struct CCWork {
     float2 *ref, *save;
     float *path, *skin, *need;
@@ -175,7 +156,6 @@ struct CCShared {
     uint32_t nout;
     uint32_t nmid, nlong;    // candidate lists sorted by a wavefront / by the workgroup
     uint32_t halt;           // the scene's time budget for this launch is spent
-    float red3[3 * 16];      // workgroup sums
     float red9[9 * 16];      // workgroup drift summary (see drift_summary)
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
 };
@@ -240,7 +220,7 @@ __device__ __forceinline__ Drift drift_summary(const float2 *P, uint32_t Np, uin
 // circles never search far.  The first `cap` partners go to out; the count
 // is returned in full.
 __device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, float2 p, float ec, double inv_h,
-                                                  uint32_t tmask, uint16_t *out, uint32_t cap) {
+                                               uint32_t tmask, uint16_t *out, uint32_t cap) {
     const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
     uint32_t kk[9];
     bool dup[9];
@@ -509,8 +489,11 @@ struct CCStats {
 // valid when no circle moved more than 0.8 of its skin since the list was
 // built (relative to the scene's mean shift): a pair not on the list started
 // further apart than the sum of the two radii and skins, so it cannot have
-// overlapped.  Otherwise the sweep is undone and rerun with larger skins for
-// the circles that moved.
+// overlapped.  When some circles went further, overruns_safe checks every
+// missing pair they belong to; only if one of them touched is the sweep
+// undone and rerun with a new list and larger skins for the circles that
+// moved.  The list is rebuilt before a sweep when a circle drifted more than
+// 0.6 of its skin since it was built.
 // Returns the new error state (0 = fine).
 template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
@@ -830,10 +813,13 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
     }
 }
 
+// One scene for `steps` updates.  MODE 2 keeps positions and previous
+// positions in LDS (s_lds), MODE 1 positions only, MODE 0 works on the scene
+// chunk in global memory.
 template <int WG, int MODE>
-__device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint32_t steps, SceneState *st, float2 *s_lds,
-                          ColliderStage *s_col,
-                          float *s_red, uint32_t *s_u32, CCShared *s_cc, uint32_t *s_wl) {
+__device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint32_t steps, SceneState *st,
+                                          float2 *s_lds, ColliderStage *s_col, float *s_red, uint32_t *s_u32,
+                                          CCShared *s_cc, uint32_t *s_wl) {
     // fields are read where used: the descriptor is uniform and read-only, so
     // they come through scalar loads instead of a private copy
     const SceneDesc &d = *dp;
