@@ -138,3 +138,13 @@ def test_mirror_to_oracle_roundtrip():
     workloads.cloth_pinned(s, n=6)
     o = to_oracle(s)
     assert bits_equal(o.get_particles()[:, :4], s.get_particles()[:, :4])
+
+
+def test_cpp_facade_program_builds_and_loads():
+    # built against include/bendy2d.hpp, the product library and the oracle;
+    # without a device it reports so (exit 2), on one it checks parity (exit 0)
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "facade_parity")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode in (0, 2), r.stdout + r.stderr
