@@ -182,7 +182,7 @@ _SIGS = [
     ("bd_ext_copy_calibration", c_int32, [c_uint64, POINTER(c_float)]),
     ("bd_batch_scene_lists", c_int32, [_S, c_int64, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
                                        POINTER(c_float), POINTER(c_float)] + [POINTER(ctypes.c_double)] * 3
-                                      + [POINTER(c_uint32)]),
+                                      + [POINTER(c_uint32)] * 2),
     ("bd_batch_scene_status", c_int32, [_S, c_int64, POINTER(c_float), POINTER(c_uint32), POINTER(c_uint32),
                                         POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]),
     ("bd_server_new", _S, [POINTER(ServerConfig)]),

@@ -470,7 +470,7 @@ int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *bui
 
 int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t *n_far, uint32_t *max_list,
                              float *s_cell, float *smax, double *t_count, double *t_lists, double *t_sort,
-                             uint32_t *kept) {
+                             uint32_t *kept, uint32_t *unchecked) {
     if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
     SceneState st{};
     if (b->uploaded && hipMemcpy(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
@@ -485,6 +485,7 @@ int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t 
     if (t_lists) *t_lists = st.t_lists * tick;
     if (t_sort) *t_sort = st.t_sort * tick;
     if (kept) *kept = st.kept;
+    if (unchecked) *unchecked = st.unchecked;
     return BD_OK;
 }
 

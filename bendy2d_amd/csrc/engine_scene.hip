@@ -424,10 +424,9 @@ __device__ __forceinline__ bool missed_pair_touches(const CCWork &w, const float
 // others hold v <= vmax, v being a circle's bound on its distance from its
 // list-build position, relative to the scene shift, over the whole sweep).
 // A pair whose list-build positions are more than r_c + r_x + v_c + v_x apart
-// stayed apart; the rest get the exact test above.  Returns whether the sweep
-// stands.
+// stayed apart; the rest get the exact test above.
 template <int WG>
-__device__ bool overruns_safe(const CCWork &w, const float *crad, const float *vb, const uint16_t *over,
+__device__ int overruns_check(const CCWork &w, const float *crad, const float *vb, const uint16_t *over,
                               uint32_t n_over, float vmax, float rmax, double h, double inv_h, uint32_t tmask,
                               float *s_red) {
     // rounding margin: a fifth of the overrun circle's skin (the margin the
@@ -437,14 +436,14 @@ __device__ bool overruns_safe(const CCWork &w, const float *crad, const float *v
         float dx = p.x - q.x, dy = p.y - q.y;
         return !(sqrtf(dx * dx + dy * dy) <= lim);  // NaN: a non-finite circle touches nothing
     };
-    float bad = 0.0f;
+    float bad = 0.0f, far = 0.0f;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t k = wave; k < n_over; k += WG / 64) {
         const uint32_t c = over[k];
         const float2 p = w.ref[c];
         const float rc = crad[c], vc = -vb[c], mc = margin(p, w.skin[c]);
         const double reach = ((double)rc + (double)vc + (double)mc + (double)rmax + (double)vmax) / h;
-        if (!(reach <= 8.0)) { bad = 1.0f; continue; }
+        if (!(reach <= 8.0)) { far = 1.0f; continue; }
         const int R = (int)ceil(reach), side = 2 * R + 1;
         const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
         for (int q = (int)lane; q < side * side; q += 64) {
@@ -468,13 +467,15 @@ __device__ bool overruns_safe(const CCWork &w, const float *crad, const float *v
         if (apart(p, w.ref[o], lim)) continue;
         if (missed_pair_touches(w, crad, c, o)) bad = 1.0f;
     }
-    return wg_max<WG>(bad, s_red) == 0.0f;
+    // 0: every missing pair stayed apart; 1: one touched; 2: an overrun too
+    // far to search (the sweep is redone without knowing)
+    return (int)wg_max<WG>(fmaxf(bad, 2.0f * far), s_red);
 }
 
 // Per-scene counters of the circle-pair schedule, kept in registers during a
 // launch (uniform across the workgroup).
 struct CCStats {
-    uint32_t rounds, redos, rebuilds, pairs, since_build, kept;
+    uint32_t rounds, redos, rebuilds, pairs, since_build, kept, unchecked;
     uint32_t n_out, n_far, max_list;
     float s_cell, smax;
     uint64_t t_build, t_count, t_lists, t_sort;
@@ -489,7 +490,7 @@ struct CCStats {
 // valid when no circle moved more than 0.8 of its skin since the list was
 // built (relative to the scene's mean shift): a pair not on the list started
 // further apart than the sum of the two radii and skins, so it cannot have
-// overlapped.  When some circles went further, overruns_safe checks every
+// overlapped.  When some circles went further, overruns_check checks every
 // missing pair they belong to; only if one of them touched is the sweep
 // undone and rerun with a new list and larger skins for the circles that
 // moved.  The list is rebuilt before a sweep when a circle drifted more than
@@ -799,12 +800,16 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         const uint32_t n_over = uni(sh->nout);
         vmax = wg_max<WG>(vmax, s_red);
         const double h = cell_size(rmax, cs.s_cell);  // the grid of the list build
-        if (n_over == 0u || (n_over <= kOutCap && overruns_safe<WG>(w, crad, vb, sh->out, n_over, vmax, rmax, h,
-                                                                   1.0 / h, tmask, s_red))) {
+        const int check = n_over == 0u ? 0
+                          : n_over > kOutCap ? 2
+                                             : overruns_check<WG>(w, crad, vb, sh->out, n_over, vmax, rmax, h, 1.0 / h,
+                                                                  tmask, s_red);
+        if (check == 0) {
             cs.since_build++;
             if (n_over) cs.kept++;
             return 0u;
         }
+        if (check == 2) cs.unchecked++;
         // undo and rebuild with the skins the circles needed
         for (uint32_t c = tid; c < Nc; c += WG) P[Np + c] = w.save[c];
         __syncthreads();
@@ -987,6 +992,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         st->t_lists += cs.t_lists;
         st->t_sort += cs.t_sort;
         st->kept += cs.kept;
+        st->unchecked += cs.unchecked;
         st->last_steps = ran;
     }
     __syncthreads();

@@ -70,6 +70,7 @@ struct SceneState {
     float s_cell, smax;
     uint64_t t_count, t_lists, t_sort;  // list build split: counts, lists, sorting (ticks)
     uint32_t kept;  // sweeps kept although circles left their skins (no missed pair touched)
+    uint32_t unchecked;  // sweeps redone because too many circles left their skins, or too far, to check
     uint32_t last_steps;  // updates the scene's last launch ran
 };
 static_assert(sizeof(SceneState) <= 128, "SceneState must fit the single-scene device blob");

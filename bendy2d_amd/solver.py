@@ -359,10 +359,10 @@ class Batch:
         u = [ctypes.c_uint32() for _ in range(3)]
         f = [ctypes.c_float() for _ in range(2)]
         d = [ctypes.c_double() for _ in range(3)]
-        k = [ctypes.c_uint32()]
+        k = [ctypes.c_uint32(), ctypes.c_uint32()]
         check(lib().bd_batch_scene_lists(self._h, int(i), *[ctypes.byref(x) for x in u + f + d + k]),
               "batch_scene_lists")
-        names = ("n_out", "n_far", "max_list", "s_cell", "smax", "t_count", "t_lists", "t_sort", "kept")
+        names = ("n_out", "n_far", "max_list", "s_cell", "smax", "t_count", "t_lists", "t_sort", "kept", "unchecked")
         return dict(zip(names, [x.value for x in u + f + d + k]))
 
     def cc_stats(self):

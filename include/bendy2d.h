@@ -323,11 +323,13 @@ int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *bui
  * above the cell threshold), of them those searched against every circle,
  * the longest list, the threshold and the largest skin; the device time so
  * far of list builds split into counting, list filling and sorting, in
- * seconds; and the sweeps kept although circles moved beyond their skins
- * (every pair missing from the lists was checked not to touch). */
+ * seconds; the sweeps kept although circles moved beyond their skins
+ * (every pair missing from the lists was checked not to touch), and those
+ * redone because too many circles, or some too far, left their skins to
+ * check. */
 int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t *n_far, uint32_t *max_list,
                              float *s_cell, float *smax, double *t_count, double *t_lists, double *t_sort,
-                             uint32_t *kept);
+                             uint32_t *kept, uint32_t *unchecked);
 
 /* ------------------------------------------------------------------ */
 /* Serving: continuous batching of scene-advance requests              */
