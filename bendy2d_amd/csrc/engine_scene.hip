@@ -149,6 +149,9 @@ constexpr uint32_t kMidList = 256;  // longer candidate lists are sorted by the 
 constexpr uint32_t kSlot = 24;      // normal partners kept from the counting pass (more are gathered again)
 static_assert(2 * kSlot <= 8 * kCandCap, "count-pass slots must fit in the trajectory array");
 constexpr uint32_t kNone = 0xffffffffu;
+// A list is rebuilt before a sweep once some circle drifted this fraction of
+// its skin from its list-build position (relative to the scene's mean shift).
+constexpr float kDriftRebuild = 1.0f;
 
 struct CCShared {
     uint16_t out[kOutCap];   // outlier circles of the current list
@@ -494,7 +497,7 @@ struct CCStats {
 // missing pair they belong to; only if one of them touched is the sweep
 // undone and rerun with a new list and larger skins for the circles that
 // moved.  The list is rebuilt before a sweep when a circle drifted more than
-// 0.6 of its skin since it was built.
+// kDriftRebuild of its skin since it was built.
 // Returns the new error state (0 = fine).
 template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
@@ -507,27 +510,29 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         bool rebuild = !list_valid;
         float2 u = make_float2(0.0f, 0.0f);
         if (!rebuild) {
-            // drift since the list was built, relative to the mean shift u.  When
-            // the displacements' bounding box lies within 0.6 of the smallest
-            // skin around u (with a margin for rounding), every circle does;
-            // otherwise each circle is checked
+            // drift since the list was built, relative to the mean shift u, as a
+            // fraction of each circle's skin (at most box / smallest skin, with
+            // a margin for rounding, when the displacements' bounding box around
+            // u is small; else each circle is measured)
             const Drift dr = drift_summary<WG>(P, Np, Nc, w, sh->red9);
             if (dr.n > 0.0f) u = make_float2(dr.sx / dr.n, dr.sy / dr.n);
             const float bx = fmaxf(fabsf(dr.xmin - u.x), fabsf(dr.xmax - u.x));
             const float by = fmaxf(fabsf(dr.ymin - u.y), fabsf(dr.ymax - u.y));
             const float box = dr.n > 0.0f ? sqrtf(bx * bx + by * by) : 0.0f;
+            float drift = (box * 1.0001f) / (dr.smin * 0.9999f);
             if (dr.nonfin > 0.0f) {
-                rebuild = true;  // a circle came back from a non-finite list-build position
-            } else if (!(box * 1.0001f <= 0.6f * dr.smin * 0.9999f)) {
-                float bad = 0.0f;
+                drift = INFINITY;  // a circle came back from a non-finite list-build position
+            } else if (!(drift <= kDriftRebuild)) {
+                float m = 0.0f;
 #pragma unroll 4
                 for (uint32_t c = tid; c < Nc; c += WG)
-                    if (moved_bound(P[Np + c], w.ref[c], u, 0.0f) > 0.6f * w.skin[c]) bad = 1.0f;
-                rebuild = wg_max<WG>(bad, s_red) > 0.0f;
+                    m = fmaxf(m, moved_bound(P[Np + c], w.ref[c], u, 0.0f) / w.skin[c]);
+                drift = wg_max<WG>(m, s_red);
             }
-            // no candidate pair at all and every circle within its skin: no pair
-            // can overlap during the sweep, which therefore changes nothing
-            if (!rebuild && cs.pairs == 0u) {
+            rebuild = !(drift <= kDriftRebuild);
+            // no candidate pair at all and every circle within 0.8 of its skin:
+            // no pair can overlap during the sweep, which therefore changes nothing
+            if (!rebuild && cs.pairs == 0u && drift <= 0.8f) {
                 cs.since_build++;
                 return 0u;
             }
