@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-drain", type=float, default=30.0, help="longest wait for the serving backlog")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--dense-at", type=int, default=300,
+                    help="also time 10 updates of the batch from this update on (0 = skip)")
     ap.add_argument("--launch", type=str, default="", help="batch launch config as JSON (skips the autotuner)")
     ap.add_argument("--only", choices=("all", "batched", "cloth", "copy"), default="all",
                     help="run one part alone (for profiler runs); only 'all' prints the JSON line")
@@ -308,6 +310,21 @@ def main():
     log(f"copy calibration: {copy}")
     if args.only == "batched":
         return
+
+    # the same batch later on, once the large softbodies have collapsed into
+    # piles in dense contact (reported beside the headline, not in it)
+    dense = None
+    if args.dense_at > bat["steps"] + args.warmup:
+        b = bat["batch"]
+        b.update(DT, args.dense_at - bat["steps"] - args.warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.update(DT, 10)
+        b.sync()
+        el = time.perf_counter() - t0
+        dense = {"updates": f"{args.dense_at + 1}-{args.dense_at + 10}", "pu_per_s": pts * 8 * 10 / el,
+                 "ms_per_update": el * 1e3 / 10, "cc": b.cc_stats()}
+        log(f"dense regime: {dense}")
 
     sample = scenes if not args.plain_sample else scenes[::max(1, len(scenes) // args.plain_sample)]
     spts = sum(s.stats()["n_points"] for s in sample)
@@ -377,6 +394,7 @@ def main():
         "plain_per_scene": {"value": plain["pu_per_s"], "scenes": len(sample),
                             "path": "bd_update per scene (drop-in), AUTO engine"},
         "batched_vs_plain": bat["pu_per_s"] / plain["pu_per_s"],
+        "dense_regime": dense,
         "cloth_1024": cloth,
         "serving": serving,
     }
