@@ -121,6 +121,53 @@ def bench_cloth(n, steps, warmup):
             "bytes_per_pu": alg / pu}
 
 
+def bench_gas(steps, warmup):
+    """262,144 free circles (r 0.5) in a 1024 box with 64 static circles and 4
+    wall polygons, circle-pair collision on, 4 substeps: the plain path."""
+    from bendy2d_amd import Solver, workloads
+    s = Solver()
+    dt, _ = workloads.gas_box(s)
+    s.update(dt)
+    for _ in range(warmup):
+        s.update(dt)
+    s.sync()
+    st0 = s.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.update(dt)
+    s.sync()
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    N, sub = st["n_points"], 4
+    pu = N * sub * steps
+    alg = 64.0 * N * sub * steps  # 32 B integrate + 32 B binning and resolve per body and substep
+    return {"pu_per_s": pu / wall, "ms_per_step": 1e3 * wall / steps, "alg_GBps": alg / wall / 1e9,
+            "frac": alg / wall / HBM_PEAK, "engine": s.engine_used, "bodies": N, "updates": steps,
+            "after_updates": 1 + warmup, "kernel_launches_per_update": (st["kernel_launches"] - st0["kernel_launches"]) / steps,
+            "cc_rounds_per_update": (st["cc_rounds"] - st0["cc_rounds"]) / steps,
+            "cc_redos": st["cc_redos"] - st0["cc_redos"], "cc_pairs": st["cc_pairs"], "cc_margin": st["cc_margin"]}
+
+
+def bench_cloth_small(steps=1000):
+    """The 32x32 pinned cloth with a static circle, 8 substeps and 8 link
+    sweeps, its whole 1,000-update run from rest on the plain path."""
+    from bendy2d_amd import Solver, workloads
+    s = Solver()
+    dt, _ = workloads.cloth_pinned(s)
+    c = s.clone()
+    c.update(dt)  # plan and upload outside the timed run
+    c.sync()
+    c.close()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.update(dt)
+    s.sync()
+    wall = time.perf_counter() - t0
+    N = s.stats()["n_points"]
+    return {"pu_per_s": N * 8 * steps / wall, "ms_per_step": 1e3 * wall / steps, "run_s": wall,
+            "engine": s.engine_used, "bodies": N, "updates": steps}
+
+
 def bench_serving(seconds, rate, launch_cfg, steps_per_launch=8, budget=64 << 30, seed=4, drain_limit=30.0,
                   launch_budget_us=20000):
     """Poisson stream of requests: half create a new softbody scene (generated
@@ -271,6 +318,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--cloth", type=int, default=1024)
+    ap.add_argument("--gas-steps", type=int, default=10,
+                    help="timed updates of the 262,144-circle gas box, plain path (0 = skip it and the 32x32 cloth)")
     ap.add_argument("--plain-sample", type=int, default=0, help="scenes for the plain path (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving run (0 = skip)")
@@ -280,7 +329,7 @@ def main():
     ap.add_argument("--dense-at", type=int, default=300,
                     help="also time 10 updates of the batch from this update on (0 = skip)")
     ap.add_argument("--launch", type=str, default="", help="batch launch config as JSON (skips the autotuner)")
-    ap.add_argument("--only", choices=("all", "batched", "cloth", "copy"), default="all",
+    ap.add_argument("--only", choices=("all", "batched", "cloth", "copy", "gas", "cloth32"), default="all",
                     help="run one part alone (for profiler runs); only 'all' prints the JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -298,6 +347,12 @@ def main():
         return
     if args.only == "cloth":
         log(f"cloth {args.cloth}^2: {bench_cloth(args.cloth, args.steps, args.warmup)}")
+        return
+    if args.only == "gas":
+        log(f"gas 262144: {bench_gas(args.steps, args.warmup)}")
+        return
+    if args.only == "cloth32":
+        log(f"cloth 32^2: {bench_cloth_small()}")
         return
 
     t0 = time.time()
@@ -334,6 +389,12 @@ def main():
     cloth = bench_cloth(args.cloth, 10, 2) if args.cloth else None
     if cloth:
         log(f"cloth {args.cloth}^2: {cloth}")
+    gas = bench_gas(args.gas_steps, 2) if args.gas_steps else None
+    if gas:
+        log(f"gas 262144: {gas}")
+    cloth32 = bench_cloth_small() if args.gas_steps else None
+    if cloth32:
+        log(f"cloth 32^2: {cloth32}")
 
     serving = None
     if args.serve_seconds > 0:
@@ -396,6 +457,8 @@ def main():
         "batched_vs_plain": bat["pu_per_s"] / plain["pu_per_s"],
         "dense_regime": dense,
         "cloth_1024": cloth,
+        "gas_262144": gas,
+        "cloth_32": cloth32,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

@@ -97,6 +97,18 @@ BD_HD bool solve_circle_pair(float &ix, float &iy, float &jx, float &jy, float r
     return false;
 }
 
+// Largest distance from a sweep's start position (sx, sy) reached so far,
+// after a push moved the circle to (px, py).  The distance is rounded up by a
+// relative 2^-20, far above float rounding, so the bound never falls below the
+// truth.  A circle pushed back and forth keeps a small excursion where the sum
+// of its pushes would grow without end.  A non-finite position leaves the
+// bound unchanged: such a circle takes part in no pair.
+BD_HD float excursion(float cur, float px, float py, float sx, float sy) {
+    float dx = px - sx, dy = py - sy;
+    float d = sqrtf(dx * dx + dy * dy) * (1.0f + 0x1p-20f);
+    return fmaxf(cur, d);
+}
+
 // Particle::update (src/particle.rs:20-25): pos + vel + acc*dt*dt, acc = g
 // (plus any initial acc on the first substep).
 BD_HD void verlet(float &px, float &py, float &qx, float &qy, float ax, float ay, float dt) {

@@ -112,16 +112,6 @@ __device__ __forceinline__ float moved_bound(float2 p, float2 r0, float2 u, floa
     return v == v ? v : INFINITY;
 }
 
-// Largest distance from the sweep's start position `s` reached so far, after
-// a push moved the circle to `p`.  The distance is rounded up by a relative
-// 2^-20, far above float rounding, so the bound never falls below the truth.
-// A circle pushed back and forth keeps a small excursion where the sum of its
-// pushes would grow without end.
-__device__ __forceinline__ float excursion(float cur, float2 p, float2 s) {
-    float dx = p.x - s.x, dy = p.y - s.y;
-    float d = sqrtf(dx * dx + dy * dy) * (1.0f + 0x1p-20f);
-    return fmaxf(cur, d);
-}
 
 __device__ __forceinline__ int scell(float x, double inv_h) {
     double c = floor((double)x * inv_h);
@@ -743,8 +733,8 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, crad[c], crad[o], qi, qj)) {
                     P[Np + c] = pi;
                     P[Np + o] = pj;
-                    w.path[c] = excursion(w.path[c], pi, w.save[c]);
-                    w.path[o] = excursion(w.path[o], pj, w.save[o]);
+                    w.path[c] = excursion(w.path[c], pi.x, pi.y, w.save[c].x, w.save[c].y);
+                    w.path[o] = excursion(w.path[o], pj.x, pj.y, w.save[o].x, w.save[o].y);
                 }
                 w.traj[oc + hc] = pi;
                 w.traj[oo + ho] = pj;
