@@ -79,6 +79,27 @@ def test_gas_dense_nan(engine):
     run_pair(s, 12, dt, check_every=4)
 
 
+def test_global_gas_hot_colliders():
+    # random overlaps turn into speed, and circles expelled from the static
+    # colliders jam into rings: candidate lists span several margin levels,
+    # components go to every replay tier (lane, LDS wavefront, workgroup), and
+    # circles leave their margins, so the per-pair check at each skipped pair's
+    # turn runs
+    s = Solver()
+    s.set_engine(BD_ENGINE_GLOBAL)
+    dt, _ = workloads.gas_box(s, n=12000, radius=0.5, box=220.0, n_static=10, seed=5)
+    s, _ = run_pair(s, 12, dt, candidate_mode=True, check_every=4, engine=BD_ENGINE_GLOBAL)
+    assert s.stats()["cc_rounds"] > 0
+
+
+def test_global_gas_pile():
+    # a dense gas falling into a pile on the floor: large components
+    s = Solver()
+    s.set_engine(BD_ENGINE_GLOBAL)
+    dt, _ = workloads.gas_box(s, n=4000, radius=0.5, box=80.0, n_static=0, seed=6)
+    run_pair(s, 60, dt, candidate_mode=True, check_every=20, engine=BD_ENGINE_GLOBAL)
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 def test_circles_links_acc(engine):
     s = Solver()
