@@ -287,6 +287,40 @@ def cpu_baseline(seconds=12.0, seed0=1000):
                       f"4096-scene workload, 1 update (8 substeps) each, {busy:.1f} s CPU"}
 
 
+def cpu_plain_runs():
+    """The f32 oracle (single thread) on the plain path's two small figures:
+    the 32x32 pinned cloth's whole 1,000-update run, and the first update of
+    the 262,144-circle gas box (the candidate-list form of the reference's
+    all-pairs loop, which it equals bit for bit)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from helpers import to_oracle
+    from bendy2d_amd import Solver, workloads
+    out = {}
+    s = Solver()
+    dt, steps = workloads.cloth_pinned(s)
+    o = to_oracle(s, 32)
+    t0 = time.process_time()
+    for _ in range(steps):
+        o.update(dt)
+    t = time.process_time() - t0
+    out["cloth_32"] = {"value": 1024 * 8 * steps / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+                       "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off), 1 thread: the whole {steps}-update "
+                                 f"run, {t:.2f} s CPU"}
+    s = Solver()
+    dt, _ = workloads.gas_box(s)
+    o = to_oracle(s, 32, candidate_mode=True)
+    n = s.stats()["n_points"]
+    t0 = time.process_time()
+    o.update(dt)
+    t = time.process_time() - t0
+    out["gas_262144"] = {"value": n * 4 / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+                         "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off, candidate lists), 1 thread: "
+                                   f"update 1 (4 substeps), {t:.2f} s CPU"}
+    s.close()
+    return out
+
+
 def profiled_traffic_per_update():
     """HBM bytes per update of the 4,096-scene batch, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel);
@@ -389,7 +423,7 @@ def main():
     cloth = bench_cloth(args.cloth, 10, 2) if args.cloth else None
     if cloth:
         log(f"cloth {args.cloth}^2: {cloth}")
-    gas = bench_gas(args.gas_steps, 2) if args.gas_steps else None
+    gas = bench_gas(args.gas_steps, 1) if args.gas_steps else None
     if gas:
         log(f"gas 262144: {gas}")
     cloth32 = bench_cloth_small() if args.gas_steps else None
@@ -410,6 +444,13 @@ def main():
         cpu["cpu_model"] = cpu_model()
         cpu["host_cpus"] = os.cpu_count()
         log(f"cpu: {cpu}")
+    if cpu and (gas or cloth32):
+        plain_cpu = cpu_plain_runs()
+        if gas:
+            gas["cpu_baseline"] = plain_cpu["gas_262144"]
+        if cloth32:
+            cloth32["cpu_baseline"] = plain_cpu["cloth_32"]
+        log(f"cpu (plain path figures): {plain_cpu}")
 
     # dominant kernel: one launch (graph replay) of k_scenes, both size classes
     # running concurrently on their streams, timed by HIP events on the batch's
