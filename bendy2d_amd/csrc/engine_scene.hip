@@ -490,7 +490,8 @@ struct CCStats {
 // kDriftRebuild of its skin since it was built.
 // Returns the new error state (0 = fine).
 template <int WG>
-__device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, float rmax, uint32_t T,
+__device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r, float runi,
+                                 float rmax, uint32_t T,
                                  const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, uint32_t *s_wl,
                                  float &sbase, bool &list_valid, CCStats &cs) {
     const int tid = threadIdx.x;
@@ -730,7 +731,8 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 const uint32_t xc = hc + 1u < w.cnt[c] ? (uint32_t)w.cand[oc + hc + 1u] : kNone;
                 const uint32_t xo = ho + 1u < w.cnt[o] ? (uint32_t)w.cand[oo + ho + 1u] : kNone;
                 float qi = 0.0f, qj = 0.0f;
-                if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, crad[c], crad[o], qi, qj)) {
+                const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
+                if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
                     P[Np + c] = pi;
                     P[Np + o] = pj;
                     w.path[c] = excursion(w.path[c], pi.x, pi.y, w.save[c].x, w.save[c].y);
@@ -835,6 +837,10 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     if constexpr (MODE == 2) Q = s_lds + d.N; else Q = gprev;
     const uint32_t N = d.N, Np = d.Np, Nc = d.Nc;
     const int tid = threadIdx.x;
+    // circle radius: the descriptor's common one when all circles share it
+    const bool uni_r = (d.flags & kFlagUniformR) != 0;
+    const float runi = d.runi;
+    auto radius = [&](uint32_t c) { return uni_r ? runi : crad[c]; };
 
     // stage colliders
     const bool staged = d.ncol <= (uint32_t)kStageCol && d.nplanes <= (uint32_t)kStagePlanes;
@@ -898,7 +904,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                         uint32_t a = rec.a & 0x7fffu, b = rec.b & 0x7fffu;
                         float2 A = P[a], B = P[b];
                         if (a >= Np) {
-                            solve_circle_link(A.x, A.y, B.x, B.y, rec.t, crad[a - Np], crad[b - Np]);
+                            solve_circle_link(A.x, A.y, B.x, B.y, rec.t, radius(a - Np), radius(b - Np));
                         } else {
                             solve_particle_link(A.x, A.y, B.x, B.y, rec.t, (rec.a & kScenePinBit) != 0,
                                                 (rec.b & kScenePinBit) != 0);
@@ -914,7 +920,8 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             if (Nc >= 2 && !err)
             {
                 const uint64_t t0 = wall_clock64();
-                err = circle_sweep<WG>(P, Np, Nc, crad, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl, skin, list_valid, cs);
+                err = circle_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl, skin,
+                                       list_valid, cs);
                 t_sweep += wall_clock64() - t0;
             }
             // ---- bodies: colliders -> bounds -> Verlet
@@ -925,7 +932,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             for (uint32_t j = tid; j < N; j += WG) {
                 if (!(pins && j < Np && ((pinmask[j >> 5] >> (j & 31)) & 1u))) {
                     float2 p = P[j];
-                    float r = j >= Np ? crad[j - Np] : 0.0f;
+                    float r = j >= Np ? radius(j - Np) : 0.0f;
                     static_colliders(p, r, d.ncol, hdr, geo, pl);
                     float2 q = Q[j];
                     float ax = d.gex, ay = d.gey;
@@ -1175,6 +1182,14 @@ int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::s
     img.acc_pending = s.any_initial_acc();
     d.rmax = 0.0f;
     for (float r : s.c_r) d.rmax = std::max(d.rmax, r);
+    // one radius for every circle (same bits): the kernel reads it from the
+    // descriptor instead of crad[]
+    d.runi = 0.0f;
+    if (!s.c_r.empty() &&
+        std::all_of(s.c_r.begin(), s.c_r.end(), [&](float r) { return std::memcmp(&r, &s.c_r[0], 4) == 0; })) {
+        d.flags |= kFlagUniformR;
+        d.runi = s.c_r[0];
+    }
     img.skin0 = std::max(0.25f * d.rmax, 0.025f);  // per circle: a pair's margin is the sum of both
     return BD_OK;
 }

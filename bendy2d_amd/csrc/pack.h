@@ -45,12 +45,13 @@ struct SceneDesc {
     uint32_t budget;             // device wall-clock ticks after which the scene stops at the
                                  // end of an update (0: run all `steps`); SceneState::last_steps
                                  // tells how many ran
-    uint32_t pad;
+    float runi;                  // every circle's radius, when kFlagUniformR is set
 };
 
 constexpr uint32_t kFlagPins = 1u;        // some particle is pinned
 constexpr uint32_t kFlagAcc = 2u;         // initial circle acc pending (first substep)
 constexpr uint32_t kFlagLds = 4u;         // positions held in LDS
+constexpr uint32_t kFlagUniformR = 8u;    // all circles share one radius (runi)
 
 // Per-scene state that persists between launches (written by the kernel).
 struct SceneState {
