@@ -30,14 +30,19 @@ __device__ __forceinline__ void static_colliders(float2 &p, float r, uint32_t nc
             float4 g = geo[k];
             static_circle(p.x, p.y, r, g.x, g.y, g.z);
         } else {
+            // the push uses the plane of largest signed distance, and only when
+            // that distance is below r; a plane at r or beyond already rules
+            // the push out, so the scan stops there (a NaN distance, first or
+            // later, never pushes either way)
             float smax = 0.0f;
             int arg = -1;
             for (int e = 0; e < h.z; e++) {
                 float4 w = pl[h.y + e];
                 float sd = (w.x * p.x + w.y * p.y) - w.z;
+                if (sd >= r) { arg = -1; break; }
                 if (arg < 0 || sd > smax) { smax = sd; arg = e; }
             }
-            if (smax < r) {
+            if (arg >= 0 && smax < r) {
                 float4 w = pl[h.y + arg];
                 float push = r - smax;
                 p.x = p.x + w.x * push; p.y = p.y + w.y * push;
