@@ -63,6 +63,19 @@ BD_HD void solve_particle_link_free(float &ax, float &ay, float &bx, float &by, 
     bx = bx + cx; by = by + cy;
 }
 
+// CircleLink::solve (src/link.rs:36-48) with ra*ra, rb*rb and
+// 1.0f / (ra*ra + rb*rb) computed once by the caller: the same f32 values, so
+// the same results as solve_circle_link.
+BD_HD void solve_circle_link_w(float &ax, float &ay, float &bx, float &by, float t, float ra2, float rb2,
+                               float scale) {
+    float dx = ax - bx, dy = ay - by;
+    float dist = sqrtf(dx * dx + dy * dy);
+    float nx = dx / dist, ny = dy / dist;
+    float e = dist - t;
+    ax = ax - ((nx * e) * scale) * rb2; ay = ay - ((ny * e) * scale) * rb2;
+    bx = bx + ((nx * e) * scale) * ra2; by = by + ((ny * e) * scale) * ra2;
+}
+
 // CircleLink::solve (src/link.rs:36-48).
 BD_HD void solve_circle_link(float &ax, float &ay, float &bx, float &by, float t, float ra, float rb) {
     float dx = ax - bx, dy = ay - by;

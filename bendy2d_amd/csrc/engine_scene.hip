@@ -841,6 +841,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     const bool uni_r = (d.flags & kFlagUniformR) != 0;
     const float runi = d.runi;
     auto radius = [&](uint32_t c) { return uni_r ? runi : crad[c]; };
+    const float r2u = runi * runi, scaleu = 1.0f / (r2u + r2u);
 
     // stage colliders
     const bool staged = d.ncol <= (uint32_t)kStageCol && d.nplanes <= (uint32_t)kStagePlanes;
@@ -906,7 +907,8 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                         uint32_t a = rec.a & 0x7fffu, b = rec.b & 0x7fffu;
                         float2 A = P[a], B = P[b];
                         if (a >= Np) {
-                            solve_circle_link(A.x, A.y, B.x, B.y, rec.t, radius(a - Np), radius(b - Np));
+                            if (uni_r) solve_circle_link_w(A.x, A.y, B.x, B.y, rec.t, r2u, r2u, scaleu);
+                            else solve_circle_link(A.x, A.y, B.x, B.y, rec.t, crad[a - Np], crad[b - Np]);
                         } else {
                             solve_particle_link(A.x, A.y, B.x, B.y, rec.t, (rec.a & kScenePinBit) != 0,
                                                 (rec.b & kScenePinBit) != 0);
