@@ -542,7 +542,11 @@ int32_t bd_batch_autotune(bd_batch *b, float dt, uint32_t steps) {
                 cf.split_points = split; cf.wg_small = ws; cf.wg_large = wl;
                 cf.steps_per_launch = 1;
                 float ms = trial(cf);
-                if (ms < best_ms) { best_ms = ms; best = cf; }
+                // a larger class split has to win by more than 1 %: it puts more
+                // mid-size scenes on the small workgroups, which fall behind once
+                // the scenes come into dense contact
+                const float margin = cf.split_points > best.split_points ? 0.99f : 1.0f;
+                if (ms < best_ms * margin) { best_ms = ms; best = cf; }
             }
     // 2) LDS budget (residency) per class
     const uint32_t lds_opts[] = {16 * 1024, 32 * 1024, 64 * 1024, 144 * 1024};
