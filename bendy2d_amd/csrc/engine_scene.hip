@@ -821,7 +821,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
 template <int WG, int MODE>
 __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint32_t steps, SceneState *st,
                                           float2 *s_lds, ColliderStage *s_col, float *s_red, uint32_t *s_u32,
-                                          CCShared *s_cc, uint32_t *s_wl) {
+                                          CCShared *s_cc, uint32_t *s_wl, uint32_t lds_bytes) {
     // fields are read where used: the descriptor is uniform and read-only, so
     // they come through scalar loads instead of a private copy
     const SceneDesc &d = *dp;
@@ -852,13 +852,18 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         for (uint32_t k = tid; k < d.ncol; k += WG) { s_col->hdr[k] = ghdr[k]; s_col->geo[k] = ggeo[k]; }
         for (uint32_t k = tid; k < d.nplanes; k += WG) s_col->pl[k] = gpl[k];
     }
-    // load state into LDS
+    // load state into LDS; with positions and previous positions there, the
+    // link records follow them when they fit too
     if constexpr (MODE >= 1) {
         for (uint32_t j = tid; j < N; j += WG) {
             P[j] = gpos[j];
             if constexpr (MODE == 2) Q[j] = gprev[j];
         }
     }
+    const bool links_lds = MODE == 2 && 16ull * N + 8ull * d.L <= lds_bytes;
+    LinkS *const s_links = (LinkS *)(s_lds + 2 * N);
+    if (links_lds)
+        for (uint32_t l = tid; l < d.L; l += WG) s_links[l] = links[l];
     const bool pins = (d.flags & kFlagPins) != 0;
     __syncthreads();
 
@@ -896,6 +901,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         for (uint32_t sub = 0; sub < d.sub_steps; sub++) {
             // ---- links: dependency layers, k sweeps
             const uint64_t tl0 = wall_clock64();
+            auto link_sweeps = [&](const LinkS *lk) {
             for (uint32_t it = 0; it < d.link_iters; it++) {
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = loff[L], hi = loff[L + 1];
@@ -903,7 +909,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                     // (register pressure of the whole persistent kernel)
 #pragma unroll 1
                     for (uint32_t l = lo + tid; l < hi; l += WG) {
-                        LinkS rec = links[l];
+                        LinkS rec = lk[l];
                         uint32_t a = rec.a & 0x7fffu, b = rec.b & 0x7fffu;
                         float2 A = P[a], B = P[b];
                         if (a >= Np) {
@@ -919,6 +925,9 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                     __syncthreads();
                 }
             }
+            };
+            if (links_lds) link_sweeps(s_links);
+            else link_sweeps(links);
             t_links += wall_clock64() - tl0;
             // ---- circle pairs
             if (Nc >= 2 && !err)
@@ -1026,11 +1035,11 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
         const uint32_t steps = d->steps < launch_steps ? d->steps : launch_steps;
         if (steps == 0 || d->N == 0) continue;
         if (16ull * d->N <= lds_bytes)
-            run_scene<WG, 2>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl);
+            run_scene<WG, 2>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl, lds_bytes);
         else if (8ull * d->N <= lds_bytes)
-            run_scene<WG, 1>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl);
+            run_scene<WG, 1>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl, lds_bytes);
         else
-            run_scene<WG, 0>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl);
+            run_scene<WG, 0>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl, lds_bytes);
     }
 }
 
@@ -1239,7 +1248,9 @@ public:
         int want = 64;
         while (want < 1024 && (uint32_t)want * 2 < desc_.N) want <<= 1;
         variant_ = variant_for_wg(want);
-        lds_bytes_ = std::min<size_t>(16ull * desc_.N, scene_max_lds());
+        // positions, previous positions and, when they fit, the link records
+        const size_t with_links = 16ull * desc_.N + 8ull * desc_.L;
+        lds_bytes_ = with_links <= scene_max_lds() ? with_links : std::min<size_t>(16ull * desc_.N, scene_max_lds());
         BD_HIP(hipStreamSynchronize(stream_));
         N_ = p.N; Np_ = p.Np;
         return BD_OK;
