@@ -1350,7 +1350,7 @@ private:
         st.launches++;
     }
 
-    static constexpr uint32_t kCompGrid = 256, kLdsGrid = 4096;
+    static constexpr uint32_t kCompGrid = 256, kLdsGrid = 16384;
     hipStream_t stream_;
     uint32_t Np_ = 0, Nc_ = 0, Ng_ = 0, N_ = 0, P_ = 0, depth_ = 0, ncol_ = 0, nplanes_ = 0;
     bool has_pins_ = false;

@@ -271,7 +271,7 @@ bool tiled_engine_fits(const HostScene &s, const Plan &p) {
 }
 
 Engine *make_tiled_engine(hipStream_t s, uint32_t tile_points, uint32_t wg) {
-    return new TiledEngine(s, tile_points ? tile_points : 4096, wg ? wg : 1024);
+    return new TiledEngine(s, tile_points ? tile_points : 2048, wg ? wg : 1024);
 }
 
 }  // namespace bd

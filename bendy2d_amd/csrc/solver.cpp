@@ -57,7 +57,9 @@ struct bd_solver {
     std::unique_ptr<Engine> eng;
     int32_t engine_pref = BD_ENGINE_AUTO;
     int32_t engine_used = 0;
-    uint32_t tile_points = 4096, tile_wg = 1024;
+    // TILED shape: 2,048-point tiles, two workgroups of 1,024 per CU on the
+    // 1024x1024 cloth (measured fastest of 1,024..8,192 points x 256..1,024 lanes)
+    uint32_t tile_points = 2048, tile_wg = 1024;
     hipStream_t stream = nullptr;
     int device = -1;
     Stats stats;

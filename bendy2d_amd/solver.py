@@ -185,7 +185,7 @@ class Solver:
     def set_engine(self, engine):
         check(lib().bd_ext_set_engine(self._h, int(engine)), "set_engine")
 
-    def set_tiling(self, tile_points=4096, workgroup=1024):
+    def set_tiling(self, tile_points=2048, workgroup=1024):
         check(lib().bd_ext_set_tiling(self._h, int(tile_points), int(workgroup)), "set_tiling")
 
     def tiled_selfcheck(self, tile_points=4096, region_cap=16384):

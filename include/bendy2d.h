@@ -223,7 +223,7 @@ int32_t bd_ext_set_pinned_range(bd_solver *s, uint64_t first, const uint8_t *pin
 int32_t bd_ext_set_engine(bd_solver *s, int32_t engine);
 int32_t bd_ext_get_engine_used(const bd_solver *s);
 /* TILED engine shape: interior points per tile (64..16384) and workgroup
- * size (256, 512 or 1024).  Defaults 4096 and 1024. */
+ * size (256, 512 or 1024).  Defaults 2048 and 1024. */
 int32_t bd_ext_set_tiling(bd_solver *s, uint32_t tile_points, uint32_t workgroup);
 /* Host check of the TILED schedule for this scene: builds the tile plan
  * (region_cap points per tile region at most), runs one link sweep of it on
