@@ -163,12 +163,12 @@ struct Drift {
 
 template <int WG>
 __device__ __forceinline__ Drift drift_summary(const float2 *P, uint32_t Np, uint32_t Nc, const CCWork &w,
-                                               float *s_red9) {
-    Drift d{0.0f, 0.0f, 0.0f, INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, 0.0f};
+                                               float *s_red9, float smin) {
+    // skins change only when the list is built, which keeps their minimum
+    Drift d{0.0f, 0.0f, 0.0f, INFINITY, -INFINITY, INFINITY, -INFINITY, smin, 0.0f};
 #pragma unroll 2
     for (uint32_t c = threadIdx.x; c < Nc; c += WG) {
         const float2 p = P[Np + c], r0 = w.ref[c];
-        d.smin = fminf(d.smin, w.skin[c]);
         if (!(fabsf(p.x) <= 3.0e38f) || !(fabsf(p.y) <= 3.0e38f)) continue;  // touches nothing
         const float dx = p.x - r0.x, dy = p.y - r0.y;
         if (fabsf(dx) <= 3.0e38f && fabsf(dy) <= 3.0e38f) {
@@ -470,7 +470,7 @@ __device__ int overruns_check(const CCWork &w, const float *crad, const float *v
 struct CCStats {
     uint32_t rounds, redos, rebuilds, pairs, since_build, kept, unchecked;
     uint32_t n_out, n_far, max_list;
-    float s_cell, smax;
+    float s_cell, smax, smin;
     uint64_t t_build, t_count, t_lists, t_sort;
 };
 
@@ -505,7 +505,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             // fraction of each circle's skin (at most box / smallest skin, with
             // a margin for rounding, when the displacements' bounding box around
             // u is small; else each circle is measured)
-            const Drift dr = drift_summary<WG>(P, Np, Nc, w, sh->red9);
+            const Drift dr = drift_summary<WG>(P, Np, Nc, w, sh->red9, cs.smin);
             if (dr.n > 0.0f) u = make_float2(dr.sx / dr.n, dr.sy / dr.n);
             const float bx = fmaxf(fabsf(dr.xmin - u.x), fabsf(dr.xmax - u.x));
             const float by = fmaxf(fabsf(dr.ymin - u.y), fabsf(dr.ymax - u.y));
@@ -535,16 +535,18 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             // s_cell: the skin at most kOutCap circles exceed (log2 histogram)
             if (tid <= 32) sh->hist[tid] = 0u;
             __syncthreads();
-            float smax = 0.0f;
+            float smax = 0.0f, nsmin = -INFINITY;
             for (uint32_t c = tid; c < Nc; c += WG) {
                 float sk = fmaxf(sbase, w.need[c]);
                 w.skin[c] = sk;
                 smax = fmaxf(smax, sk);
+                nsmin = fmaxf(nsmin, -sk);
                 int b = 0;
                 if (sk > sbase) b = sk <= 3.0e38f ? min(32, 1 + (int)floorf(log2f(sk / sbase))) : 32;
                 atomicAdd(&sh->hist[b], 1u);
             }
             smax = wg_max<WG>(smax, s_red);
+            cs.smin = -wg_max<WG>(nsmin, s_red);
             int B = 0;
             {
                 uint32_t above = 0;
@@ -890,7 +892,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     cs.pairs = uni(st->pairs);
     cs.since_build = uni(st->since_build);
     cs.n_out = uni(st->n_out); cs.n_far = uni(st->n_far); cs.max_list = uni(st->max_list);
-    cs.s_cell = uni(st->s_cell); cs.smax = uni(st->smax);
+    cs.s_cell = uni(st->s_cell); cs.smax = uni(st->smax); cs.smin = uni(st->smin);
     uint64_t t_links = 0, t_sweep = 0, t_bodies = 0;
     bool acc = (d.flags & kFlagAcc) != 0;
     const float2 *acc0 = (const float2 *)(base + d.acc0_off);
@@ -1003,6 +1005,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         st->max_list = cs.max_list;
         st->s_cell = cs.s_cell;
         st->smax = cs.smax;
+        st->smin = cs.smin;
         st->t_count += cs.t_count;
         st->t_lists += cs.t_lists;
         st->t_sort += cs.t_sort;

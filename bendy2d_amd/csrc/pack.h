@@ -73,6 +73,7 @@ struct SceneState {
     uint32_t kept;  // sweeps kept although circles left their skins (no missed pair touched)
     uint32_t unchecked;  // sweeps redone because too many circles left their skins, or too far, to check
     uint32_t last_steps;  // updates the scene's last launch ran
+    float smin;           // smallest skin of the last candidate list
 };
 static_assert(sizeof(SceneState) <= 128, "SceneState must fit the single-scene device blob");
 
