@@ -100,6 +100,24 @@ def test_global_gas_pile():
     run_pair(s, 60, dt, candidate_mode=True, check_every=20, engine=BD_ENGINE_GLOBAL)
 
 
+def test_global_overlap_chain():
+    # a row of circles each overlapping the next by half a radius: every pair
+    # of the row is in contact, so one sweep pushes every circle far beyond
+    # the base margin (overruns, the per-pair check, margins on several
+    # levels), and one component spans the whole row
+    s = Solver()
+    s.set_engine(BD_ENGINE_GLOBAL)
+    s.gravity = (0.0, 0.0)
+    s.bounds = ((-5000.0, -5000.0), (10000.0, 10000.0))
+    for k in range(300):
+        p = np.array([0.5 * k, 0.01 * (k % 7)], np.float32)
+        s.add_circle(p, 0.5)
+    for k in range(40):  # a few quiet circles around, on the base level
+        p = np.array([-30.0 + 1.5 * k, 20.0], np.float32)
+        s.add_circle(p, 0.5)
+    run_pair(s, 6, 1.0 / 60.0, check_every=2, engine=BD_ENGINE_GLOBAL)
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 def test_circles_links_acc(engine):
     s = Solver()
