@@ -187,7 +187,8 @@ def bench_serving(seconds, rate, launch_cfg, steps_per_launch=64, budget=64 << 3
     ks = rng.integers(10, 601, len(arrivals))
     picks = rng.random(len(arrivals))
     srv = Server(memory_budget=budget, steps_per_launch=steps_per_launch, launch_budget_us=launch_budget_us,
-                 launch={k: launch_cfg[k] for k in ("wg_small", "wg_large", "split_points", "lds_small", "lds_large")})
+                 launch={k: launch_cfg[k] for k in ("wg_small", "wg_large", "split_points", "lds_small", "lds_large",
+                                                    "split_tiny", "wg_tiny", "lds_tiny")})
     known, lag, submitted, done = [], [], {}, []
     t0 = time.perf_counter()
     last_log = 0.0

@@ -69,7 +69,7 @@ class GridDesc(Structure):
 class BatchConfig(Structure):
     _fields_ = [("wg_small", c_uint32), ("wg_large", c_uint32), ("split_points", c_uint32),
                 ("lds_small", c_uint32), ("lds_large", c_uint32), ("steps_per_launch", c_uint32),
-                ("use_graph", c_int32)]
+                ("use_graph", c_int32), ("split_tiny", c_uint32), ("wg_tiny", c_uint32), ("lds_tiny", c_uint32)]
 
 
 class ServerConfig(Structure):

@@ -19,15 +19,15 @@ namespace bd {
 struct BatchCore {
     bd_batch_config cfg{};
     std::vector<SceneDesc> descs;     // indexed by scene slot
-    std::vector<uint32_t> order;      // small class then large class, largest first
-    uint32_t n_small = 0, n_large = 0;
+    std::vector<uint32_t> order;      // tiny, small, then large class, each largest first
+    uint32_t n_tiny = 0, n_small = 0, n_large = 0;
     bool descs_dirty = true, order_dirty = true;
     DevBuf<uint8_t> d_desc, d_order, d_queue;
     size_t d_desc_cap = 0;
     SceneState *states_ptr = nullptr;
-    hipStream_t stream = nullptr, side = nullptr;
+    hipStream_t stream = nullptr, side = nullptr, side2 = nullptr;
     bool own_stream = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_steps = 0;
     int num_cu = 256;

@@ -304,10 +304,12 @@ bd_server *bd_server_new(const bd_server_config *cfg) {
     std::unique_ptr<bd_server> S(new bd_server());
     S->cfg = *cfg;
     S->core.cfg = cfg->launch;
-    if (!S->core.cfg.wg_small) S->core.cfg = bd_batch_config{128, 1024, 2048, 32 * 1024, 144 * 1024, 1, 0};
+    if (!S->core.cfg.wg_small) S->core.cfg = bd_batch_config{128, 1024, 2048, 32 * 1024, 144 * 1024, 1, 0, 400, 64, 8 * 1024};
     S->core.cfg.use_graph = 0;
     S->core.cfg.steps_per_launch = std::max<uint32_t>(cfg->steps_per_launch, 1);
-    if (variant_for_wg((int)S->core.cfg.wg_small) < 0 || variant_for_wg((int)S->core.cfg.wg_large) < 0) {
+    if (S->core.cfg.lds_tiny > scene_max_lds()) S->core.cfg.lds_tiny = (uint32_t)scene_max_lds();
+    if (variant_for_wg((int)S->core.cfg.wg_small) < 0 || variant_for_wg((int)S->core.cfg.wg_large) < 0 ||
+        (S->core.cfg.split_tiny && variant_for_wg((int)S->core.cfg.wg_tiny) < 0)) {
         set_last_error("workgroup sizes must be 64, 128, 256, 512 or 1024");
         return nullptr;
     }

@@ -276,13 +276,16 @@ typedef struct bd_batch bd_batch;
  * lds_large bytes of LDS per workgroup decide residency (pos and prev in LDS
  * when 16 B per body fit, pos alone when 8 B fit, else global memory);
  * steps_per_launch updates run per kernel launch; use_graph replays the launch
- * sequence as a captured hipGraph. */
+ * sequence as a captured hipGraph.  A third class takes the smallest scenes:
+ * those of at most split_tiny bodies run on workgroups of wg_tiny threads with
+ * lds_tiny bytes of LDS (split_tiny = 0: no third class). */
 typedef struct bd_batch_config {
     uint32_t wg_small, wg_large;
     uint32_t split_points;
     uint32_t lds_small, lds_large;
     uint32_t steps_per_launch;
     int32_t use_graph;
+    uint32_t split_tiny, wg_tiny, lds_tiny;
 } bd_batch_config;
 
 bd_batch *bd_batch_new(void);

@@ -38,11 +38,19 @@ def oracle_run(scenes, steps):
     return outs
 
 
+# the tiny class (scenes of at most split_tiny bodies on their own workgroups)
+# on, off, and covering the whole small class
 CONFIGS = [
-    dict(wg_small=64, wg_large=256, split_points=256, lds_small=16384, lds_large=147456, steps_per_launch=1, use_graph=1),
-    dict(wg_small=128, wg_large=1024, split_points=2048, lds_small=32768, lds_large=65536, steps_per_launch=3, use_graph=1),
-    dict(wg_small=256, wg_large=512, split_points=100000, lds_small=4096, lds_large=4096, steps_per_launch=2, use_graph=0),
-    dict(wg_small=64, wg_large=1024, split_points=64, lds_small=0, lds_large=0, steps_per_launch=4, use_graph=0),
+    dict(wg_small=64, wg_large=256, split_points=256, lds_small=16384, lds_large=147456, steps_per_launch=1, use_graph=1,
+         split_tiny=0),
+    dict(wg_small=128, wg_large=1024, split_points=2048, lds_small=32768, lds_large=65536, steps_per_launch=3, use_graph=1,
+         split_tiny=100, wg_tiny=64, lds_tiny=8192),
+    dict(wg_small=256, wg_large=512, split_points=100000, lds_small=4096, lds_large=4096, steps_per_launch=2, use_graph=0,
+         split_tiny=400, wg_tiny=128, lds_tiny=0),
+    dict(wg_small=64, wg_large=1024, split_points=64, lds_small=0, lds_large=0, steps_per_launch=4, use_graph=0,
+         split_tiny=0),
+    dict(wg_small=256, wg_large=1024, split_points=1024, lds_small=32768, lds_large=147456, steps_per_launch=2,
+         use_graph=1, split_tiny=4096, wg_tiny=64, lds_tiny=16384),
 ]
 
 
