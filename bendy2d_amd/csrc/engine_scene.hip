@@ -907,11 +907,16 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             for (uint32_t it = 0; it < d.link_iters; it++) {
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = loff[L], hi = loff[L + 1];
-                    // not unrolled: measured faster than unrolling by 2, 4 or 8
-                    // (register pressure of the whole persistent kernel)
+                    // not unrolled (measured faster than unrolling by 2, 4 or 8:
+                    // register pressure of the whole persistent kernel); the
+                    // thread's next record loads while this one is solved
+                    uint32_t l = lo + tid;
+                    LinkS nxt{0, 0, 0.0f};
+                    if (l < hi) nxt = lk[l];
 #pragma unroll 1
-                    for (uint32_t l = lo + tid; l < hi; l += WG) {
-                        LinkS rec = lk[l];
+                    for (; l < hi; l += WG) {
+                        const LinkS rec = nxt;
+                        if (l + WG < hi) nxt = lk[l + WG];
                         uint32_t a = rec.a & 0x7fffu, b = rec.b & 0x7fffu;
                         float2 A = P[a], B = P[b];
                         if (a >= Np) {
