@@ -304,8 +304,10 @@ int32_t bd_batch_read_scene(bd_batch *b, int64_t id, bd_solver *out);
 int32_t bd_batch_set_config(bd_batch *b, const bd_batch_config *cfg);
 int32_t bd_batch_get_config(const bd_batch *b, bd_batch_config *cfg);
 /* Time candidate configurations on this batch (each for `steps` updates of
- * dt on a scratch copy of the state, which is restored afterwards) and keep
- * the fastest.  Returns the number of configurations tried. */
+ * dt on a scratch copy of the state, which is restored afterwards, after the
+ * same updates once untimed) and keep the fastest: workgroup sizes, the
+ * small/large split, LDS budgets, updates per launch.  The smallest class
+ * keeps its configured shape.  Returns the number of configurations tried. */
 int32_t bd_batch_autotune(bd_batch *b, float dt, uint32_t steps);
 /* Device time of the last bd_batch_update in milliseconds (HIP events on the
  * batch's stream around the launches), and the launches it took. */
