@@ -330,9 +330,11 @@ __global__ void __launch_bounds__(256) k_cg_prep(const float2 *__restrict__ pos,
         atomicAdd(&count[k], 1u);
         bit = 1u << L;
     }
-    // occupied levels: one atomic per wavefront
+    // occupied levels above 0 (level 0 counts as occupied): one atomic per
+    // wavefront that holds such a level, so the common all-level-0 wavefront
+    // takes none
     for (int o = 32; o > 0; o >>= 1) bit |= __shfl_xor(bit, o);
-    if ((threadIdx.x & 63u) == 0u && bit) atomicOr(&ctl[kCtlMask], bit);
+    if ((threadIdx.x & 63u) == 0u && (bit & ~1u)) atomicOr(&ctl[kCtlMask], bit);
 }
 
 __global__ void __launch_bounds__(256) k_cg_scatter(const float2 *__restrict__ save, const float *__restrict__ reach,
@@ -427,7 +429,7 @@ __global__ void __launch_bounds__(256) k_cg_cand(const float2 *__restrict__ save
     if (fabsf(p.x) <= 3.0e38f && fabsf(p.y) <= 3.0e38f) {
         const float ri = reach[i];
         const uint32_t Li = lvl[i];
-        const uint32_t mask = ctl[kCtlMask];
+        const uint32_t mask = ctl[kCtlMask] | 1u;
         const uint32_t below = mask & ((2u << Li) - 1u);  // levels <= Li
         auto test = [&](uint32_t j, float qx, float qy, float rj) {
             const float dx = p.x - qx, dy = p.y - qy, lim = ri + rj;
@@ -956,7 +958,7 @@ __global__ void __launch_bounds__(256) k_cg_verify(const float2 *__restrict__ sa
     const float2 p = save[i];
     if (!(fabsf(p.x) <= 3.0e38f && fabsf(p.y) <= 3.0e38f)) return;
     const float ai = crad[i] + exc[i];
-    const uint32_t mask = ctl[kCtlMask];
+    const uint32_t mask = ctl[kCtlMask] | 1u;
     const uint32_t *const list = cand + off[i];
     const uint32_t nl = cnt[i];
     bool missed = false;
