@@ -355,7 +355,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--cloth", type=int, default=1024)
-    ap.add_argument("--gas-steps", type=int, default=10,
+    ap.add_argument("--gas-steps", type=int, default=30,
                     help="timed updates of the 262,144-circle gas box, plain path (0 = skip it and the 32x32 cloth)")
     ap.add_argument("--plain-sample", type=int, default=0, help="scenes for the plain path (0 = all)")
     ap.add_argument("--no-cpu", action="store_true")

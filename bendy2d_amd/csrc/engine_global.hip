@@ -1339,7 +1339,8 @@ private:
             hipLaunchKernelGGL(k_cc_restore, dim3(nb), dim3(256), 0, stream_, pos_.p, Np_, cc_save_.p, nc);
             st.launches++;
             clean_ = 0;
-            if (host_ctl_[kCtlQuiet]) s0_ = std::min(1.5f * s0_, std::max(0.5f * rmax_, s0_min()));
+            // capped at 0.75 r_max: measured best over the gas box's run (0.25-1.0)
+            if (host_ctl_[kCtlQuiet]) s0_ = std::min(1.5f * s0_, std::max(0.75f * rmax_, s0_min()));
         }
         err = "circle pair margin did not converge";
         return BD_ERR_COLLISION;
