@@ -23,12 +23,25 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def use_library(path=None):
+    """Load the oracle from `path` for the calls that follow (None: the built
+    liboracle.so).  The mutation test points the known-answer tests at copies
+    of the oracle with a planted slip this way."""
+    global _lib, _lib_path
+    _lib = None
+    _lib_path = path
+
+
+_lib_path = None
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = _lib_path or LIB_PATH
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         for pfx in ("or32_", "or64_"):
             sig = {
                 "new": (c_void_p, []), "free": (None, [c_void_p]), "clone": (c_void_p, [c_void_p]),
@@ -63,6 +76,16 @@ def lib():
                 "get_polygon": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
                 "get_polygon_links": (c_int64, [c_void_p, c_int64, c_void_p, c_void_p]),
             }
+            c_real = c_float if pfx == "or32_" else c_double
+            sig.update({
+                "law_particle_link": (None, [c_void_p, c_real, c_int, c_int]),
+                "law_circle_link": (None, [c_void_p, c_real, c_real, c_real]),
+                "law_circle_pair": (c_int, [c_void_p, c_real, c_real]),
+                "law_bounds": (None, [c_void_p, c_void_p, c_int, c_real]),
+                "law_particle_update": (None, [c_void_p, c_real]),
+                "law_line_intersection": (c_int, [c_void_p, c_void_p]),
+                "law_resolve_line": (c_int, [c_void_p, c_void_p]),
+            })
             for name, (res, args) in sig.items():
                 f = getattr(L, pfx + name)
                 f.restype = res
@@ -81,10 +104,11 @@ class OracleSolver:
     def __init__(self, bits=32, _h=None):
         self.bits = bits
         self.pfx = "or32_" if bits == 32 else "or64_"
+        self._L = lib()
         self._h = _h if _h is not None else self._f("new")()
 
     def _f(self, name):
-        return getattr(lib(), self.pfx + name)
+        return getattr(self._L, self.pfx + name)
 
     def close(self):
         if self._h:
@@ -212,6 +236,72 @@ class OracleSolver:
         out = np.zeros((max(n, 1), 6), np.float64)
         c = np.zeros(2, np.float64)
         self._f("get_polygon")(self._h, int(i), _p(out), _p(c))
-        ab = np.zeros((max(2 * n * 2, 2), 2), np.int64)
-        t = np.zeros(max(2 * n * 2, 1), np.float64)
-        return {"particles": out[:n], "center": (c[0], c[1])}
+        ab = np.zeros((max(4 * n, 2), 2), np.int64)
+        t = np.zeros(max(4 * n, 1), np.float64)
+        nl = self._f("get_polygon_links")(self._h, int(i), _p(ab), _p(t))
+        return {"particles": out[:n], "center": (c[0], c[1]), "links": (ab[:nl, 0], ab[:nl, 1], t[:nl])}
+
+
+# ---- single laws (the exact functions the solver step uses), for known-answer tests ----
+
+def _real(bits):
+    return np.float32 if bits == 32 else np.float64
+
+
+def _pfx(bits):
+    return "or32_" if bits == 32 else "or64_"
+
+
+def law_particle_link(a, b, t, pinned_a=False, pinned_b=False, bits=32):
+    """ParticleLink::solve on one link; returns the new (a, b)."""
+    ab = np.array([a[0], a[1], b[0], b[1]], _real(bits))
+    getattr(lib(), _pfx(bits) + "law_particle_link")(_p(ab), t, int(pinned_a), int(pinned_b))
+    return ab[:2].copy(), ab[2:].copy()
+
+
+def law_circle_link(a, b, ra, rb, t, bits=32):
+    """CircleLink::solve on one link of circles of radii ra, rb; returns the new (a, b)."""
+    ab = np.array([a[0], a[1], b[0], b[1]], _real(bits))
+    getattr(lib(), _pfx(bits) + "law_circle_link")(_p(ab), ra, rb, t)
+    return ab[:2].copy(), ab[2:].copy()
+
+
+def law_circle_pair(a, b, ra, rb, bits=32):
+    """Circle::solve_circle on one pair; returns (hit, new a, new b)."""
+    ab = np.array([a[0], a[1], b[0], b[1]], _real(bits))
+    hit = getattr(lib(), _pfx(bits) + "law_circle_pair")(_p(ab), ra, rb)
+    return bool(hit), ab[:2].copy(), ab[2:].copy()
+
+
+def law_bounds(pos, prev, bounds_pos, bounds_size, radius=None, bits=32):
+    """Particle::solve_bounds (radius None) or Circle::solve_bounds; returns (pos, prev)."""
+    pp = np.array([pos[0], pos[1], prev[0], prev[1]], _real(bits))
+    bb = np.array([bounds_pos[0], bounds_pos[1], bounds_size[0], bounds_size[1]], _real(bits))
+    getattr(lib(), _pfx(bits) + "law_bounds")(_p(pp), _p(bb), int(radius is not None),
+                                               0.0 if radius is None else radius)
+    return pp[:2].copy(), pp[2:].copy()
+
+
+def law_particle_update(pos, prev, acc, dt, bits=32):
+    """Particle::update (Verlet); returns (pos, prev, acc)."""
+    p = np.array([pos[0], pos[1], prev[0], prev[1], acc[0], acc[1]], _real(bits))
+    getattr(lib(), _pfx(bits) + "law_particle_update")(_p(p), dt)
+    return p[:2].copy(), p[2:4].copy(), p[4:].copy()
+
+
+def law_line_intersection(p1, p2, p3, p4, bits=32):
+    """common::line_intersection of segments p1-p2 and p3-p4; the point or None."""
+    p = np.array([p1[0], p1[1], p2[0], p2[1], p3[0], p3[1], p4[0], p4[1]], _real(bits))
+    out = np.zeros(2, _real(bits))
+    hit = getattr(lib(), _pfx(bits) + "law_line_intersection")(_p(p), _p(out))
+    return out if hit else None
+
+
+def law_resolve_line(center, a, b, o, other_center, bits=32):
+    """Polygon::resolve_line_intersection of edge a-b (polygon centre `center`)
+    against point o (its polygon's centre other_center); (new_a, new_b, new_o)
+    or None."""
+    v = np.array([*center, *a, *b, *o, *other_center], _real(bits))
+    out = np.zeros(6, _real(bits))
+    hit = getattr(lib(), _pfx(bits) + "law_resolve_line")(_p(v), _p(out))
+    return (out[0:2], out[2:4], out[4:6]) if hit else None

@@ -18,6 +18,15 @@
  *   normalize(v)    = (v.x / |v|, v.y / |v|)   (two divisions)
  *   v * s, v / s    = componentwise; s * v == v * s
  * The file must be compiled with -ffp-contract=off and without fast-math.
+ *
+ * Pinning: every law is checked by exact known answers and invariants
+ * (tests/test_oracle_kat.py) and bit for bit against a second restatement
+ * written from the Rust source (tests/rust_model.py,
+ * tests/test_oracle_model.py); tests/test_oracle_mutations.py plants 31
+ * plausible slips in copies of this file and each one fails a test.  Parity
+ * unpinned: the instruction order inside nalgebra's magnitude / normalize /
+ * dot, which both restatements read from nalgebra 0.32's published source and
+ * only a rustc build of the crate could confirm.
  */
 
 #ifndef REAL
@@ -705,6 +714,76 @@ int F(apply_particle_links_in_order)(F(solver) *s, const int64_t *order, int64_t
         F(plink_solve)(&s->parts[l->a].pos, &s->parts[l->b].pos, l->t, s->pinned[l->a], s->pinned[l->b]);
     }
     return 0;
+}
+
+/* ---------------- single laws, for the known-answer and model tests ----------------
+ * Each runs exactly the function the solver step uses, on caller values, so a
+ * test can pin one law at a time.  Vectors are (x, y) pairs of REAL. */
+
+/* ParticleLink::solve (src/link.rs:78-87) with the EXT3 pin rule; ab = {ax, ay, bx, by} in/out */
+void F(law_particle_link)(REAL *ab, REAL t, int pa, int pb) {
+    T(v2) a = {ab[0], ab[1]}, b = {ab[2], ab[3]};
+    F(plink_solve)(&a, &b, t, pa, pb);
+    ab[0] = a.x; ab[1] = a.y; ab[2] = b.x; ab[3] = b.y;
+}
+
+/* CircleLink::solve (src/link.rs:96-108); ab = {ax, ay, bx, by} in/out */
+void F(law_circle_link)(REAL *ab, REAL ra, REAL rb, REAL t) {
+    T(circle) ca, cb;
+    memset(&ca, 0, sizeof ca); memset(&cb, 0, sizeof cb);
+    ca.pt.pos.x = ab[0]; ca.pt.pos.y = ab[1]; ca.r = ra;
+    cb.pt.pos.x = ab[2]; cb.pt.pos.y = ab[3]; cb.r = rb;
+    F(clink_solve)(&ca, &cb, t);
+    ab[0] = ca.pt.pos.x; ab[1] = ca.pt.pos.y; ab[2] = cb.pt.pos.x; ab[3] = cb.pt.pos.y;
+}
+
+/* Circle::solve_circle (src/circle.rs:141-154); returns 1 when the pair overlapped */
+int F(law_circle_pair)(REAL *ab, REAL ra, REAL rb) {
+    T(circle) ca, cb;
+    memset(&ca, 0, sizeof ca); memset(&cb, 0, sizeof cb);
+    ca.pt.pos.x = ab[0]; ca.pt.pos.y = ab[1]; ca.r = ra;
+    cb.pt.pos.x = ab[2]; cb.pt.pos.y = ab[3]; cb.r = rb;
+    int hit = F(circle_pair)(&ca, &cb, NULL, NULL);
+    ab[0] = ca.pt.pos.x; ab[1] = ca.pt.pos.y; ab[2] = cb.pt.pos.x; ab[3] = cb.pt.pos.y;
+    return hit;
+}
+
+/* Particle::solve_bounds (src/particle.rs:27-46) or, with is_circle, Circle::solve_bounds
+ * (src/circle.rs:120-139); pp = {pos.x, pos.y, prev.x, prev.y} in/out, b = {pos.x, pos.y, size.x, size.y} */
+void F(law_bounds)(REAL *pp, const REAL *b, int is_circle, REAL r) {
+    T(v2) bp = {b[0], b[1]}, bs = {b[2], b[3]};
+    T(circle) c;
+    memset(&c, 0, sizeof c);
+    c.pt.pos.x = pp[0]; c.pt.pos.y = pp[1]; c.pt.prev.x = pp[2]; c.pt.prev.y = pp[3]; c.r = r;
+    if (is_circle) F(circle_bounds)(&c, bp, bs); else F(particle_bounds)(&c.pt, bp, bs);
+    pp[0] = c.pt.pos.x; pp[1] = c.pt.pos.y; pp[2] = c.pt.prev.x; pp[3] = c.pt.prev.y;
+}
+
+/* Particle::update (src/particle.rs:20-25); p = {pos.x, pos.y, prev.x, prev.y, acc.x, acc.y} in/out */
+void F(law_particle_update)(REAL *p, REAL dt) {
+    T(particle) q;
+    q.pos.x = p[0]; q.pos.y = p[1]; q.prev.x = p[2]; q.prev.y = p[3]; q.acc.x = p[4]; q.acc.y = p[5];
+    F(particle_update)(&q, dt);
+    p[0] = q.pos.x; p[1] = q.pos.y; p[2] = q.prev.x; p[3] = q.prev.y; p[4] = q.acc.x; p[5] = q.acc.y;
+}
+
+/* common::line_intersection (src/common.rs:4-26); p = {p1, p2, p3, p4} as 8 REALs; 1 = Some(out) */
+int F(law_line_intersection)(const REAL *p, REAL *out) {
+    T(v2) p1 = {p[0], p[1]}, p2 = {p[2], p[3]}, p3 = {p[4], p[5]}, p4 = {p[6], p[7]}, r = {0, 0};
+    int hit = F(line_intersection)(p1, p2, p3, p4, &r);
+    out[0] = r.x; out[1] = r.y;
+    return hit;
+}
+
+/* Polygon::resolve_line_intersection (src/polygon.rs:164-216) for an edge a-b
+ * of a polygon with centre c against a point o of a polygon with centre oc:
+ * v = {c, a, b, o, oc} as 10 REALs; out = {new_a, new_b, new_o}; 1 = Some */
+int F(law_resolve_line)(const REAL *v, REAL *out) {
+    T(v2) c = {v[0], v[1]}, a = {v[2], v[3]}, b = {v[4], v[5]}, o = {v[6], v[7]}, oc = {v[8], v[9]};
+    T(v2) na = {0, 0}, nb = {0, 0}, no = {0, 0};
+    int hit = F(resolve_line)(c, a, b, o, oc, &na, &nb, &no);
+    out[0] = na.x; out[1] = na.y; out[2] = nb.x; out[3] = nb.y; out[4] = no.x; out[5] = no.y;
+    return hit;
 }
 
 /* ---------------- readback (values widened to double, exact for f32) ---------------- */

@@ -1,13 +1,20 @@
-"""Known-answer tests that pin the f32 oracle without the reference crate.
+"""Known-answer tests that pin the f32 oracle by what the mathematics fixes.
 
-The crate ships no tests, fixtures or golden vectors, and cannot be built here
-(no rustc, no nalgebra source), so each oracle function is pinned by what the
-mathematics fixes: exact closed forms chosen so f32 arithmetic is exact,
-invariants (midpoint, r^2-weighted centroid, reflected velocity), textbook
-cases of segment intersection, and brute-force agreement between two
-independent implementations (candidate-list sweep vs the reference's all-pairs
-loop).  A plausible slip (a dropped term, a swapped index or sign, acc*dt
-instead of acc*dt*dt, a reciprocal instead of a division) fails at least one.
+The crate ships no tests, fixtures or golden vectors and cannot be built here
+(no rustc, no nalgebra source).  So each oracle law is pinned by:
+  - exact closed forms, with inputs chosen so every f32 operation is exact and
+    the expected value is the real-number answer;
+  - IEEE-754 rounding facts (ties to even, correctly rounded division) where an
+    operation order must be pinned;
+  - invariants the law preserves (midpoint, r^2-weighted centroid, reflected
+    velocity, chord lengths of a regular polygon);
+  - textbook special cases of segment intersection (parallel, collinear,
+    endpoints at s and t equal to 0 and 1);
+  - brute force: the candidate-list circle sweep against the all-pairs loop.
+tests/test_oracle_model.py adds a second restatement written from the Rust
+source, compared bit for bit; tests/test_oracle_mutations.py plants plausible
+slips in copies of the oracle and checks that one of these tests fails for
+each.
 """
 import math
 
@@ -15,9 +22,17 @@ import numpy as np
 import pytest
 
 from helpers import bits_equal
-from oracle import OracleSolver
+from oracle import (OracleSolver, law_bounds, law_circle_link, law_circle_pair, law_line_intersection,
+                    law_particle_link, law_particle_update)
 
+f32 = np.float32
 BIG = ((-1e6, -1e6), (2e6, 2e6))
+
+
+@pytest.fixture(autouse=True)
+def _quiet_fp():
+    with np.errstate(all="ignore"):
+        yield
 
 
 def solver(bits=32, g=(0.0, 0.0), bounds=BIG):
@@ -27,153 +42,507 @@ def solver(bits=32, g=(0.0, 0.0), bounds=BIG):
     return o
 
 
+def exact(got, want):
+    return bits_equal(np.asarray(got, np.float32).ravel(), np.asarray(want, np.float32).ravel())
+
+
+# ---------------------------------------------------------------- Solver::new, update
+
 def test_defaults_match_solver_new():
-    # Solver::new (src/solver.rs:34-50): gravity (0, 98.2), bounds 100x100.
+    # Solver::new (src/solver.rs:34-50): gravity (0, 98.2), bounds (0,0)+(100,100).
+    # One update of dt = 1 from rest at (50, 50): bounds run before integration,
+    # so the particle ends at 50 + 98.2 (not clamped) with prev 50.
     o = OracleSolver(32)
     o.add_particle((50.0, 50.0))
     o.update(1.0)
     p = o.get_particles()[0]
-    # acc = 98.2 (f32), one step: pos = pos + 0 + acc*1*1
-    assert p[1] == np.float32(50.0) + np.float32(98.2) or p[1] == 100.0  # then clamped below
-    # the particle crossed the lower-right bound (y > 100) only after integration,
-    # so it is not clamped in this update (bounds run before integrate)
-    assert p[1] == np.float32(np.float32(50.0) + np.float32(98.2))
-    assert p[3] == 50.0
-
-
-def test_single_link_known_answer():
-    # one link (0,0)-(2,0), target 1: each end moves 0.5 inward, exactly
-    o = solver()
-    o.add_particle((0.0, 0.0))
-    o.add_particle((2.0, 0.0))
-    o.add_particle_link(0, 1, 1.0)
-    o.apply_particle_links_in_order([0])
-    p = o.get_particles()
-    assert p[0, 0] == 0.5 and p[1, 0] == 1.5 and p[0, 1] == 0 and p[1, 1] == 0
-
-
-def test_link_preserves_midpoint_and_reaches_target():
-    rng = np.random.default_rng(1)
-    for _ in range(200):
-        a, b = rng.uniform(-10, 10, 2), rng.uniform(-10, 10, 2)
-        t = float(rng.uniform(0.1, 5))
-        o = solver(64)
-        o.add_particle(a)
-        o.add_particle(b)
-        o.add_particle_link(0, 1, t)
-        o.apply_particle_links_in_order([0])
-        p = o.get_particles()
-        mid0 = (np.float32(a).astype(np.float64) + np.float32(b)) / 2
-        assert np.allclose((p[0, :2] + p[1, :2]) / 2, mid0, atol=1e-12)
-        assert abs(np.linalg.norm(p[0, :2] - p[1, :2]) - np.float32(t)) < 1e-9
-
-
-def test_pinned_link_moves_only_free_end():
-    o = solver()
-    o.add_particle((0.0, 0.0))
-    o.add_particle((3.0, 4.0))
-    o.pin(0)
-    o.add_particle_link(0, 1, 1.0)
-    o.apply_particle_links_in_order([0])
-    p = o.get_particles()
-    assert p[0, 0] == 0 and p[0, 1] == 0
-    # free end lands on the target circle around the pinned end: (0.6, 0.8)
-    assert abs(p[1, 0] - 0.6) < 1e-6 and abs(p[1, 1] - 0.8) < 1e-6
+    assert exact(p[:4], (50.0, f32(50.0) + f32(98.2), 50.0, 50.0))
+    # next update: y = 148.2 > 100 is clamped to 100 with its velocity 98.2
+    # reflected (prev = 100 + 98.2), then integrated: v = 100 - 198.2
+    o.update(1.0)
+    p = o.get_particles()[0]
+    vy = f32(f32(50.0) + f32(98.2)) - f32(50.0)
+    prev_y = f32(100.0) + vy
+    assert exact(p[1:4:2], (f32(f32(100.0) + f32(100.0 - prev_y)) + f32(98.2), 100.0))
 
 
 def test_free_fall_exact_closed_form():
-    # g = -8, dt = 1/16 (1 substep): acc*dt*dt = -1/32 exactly, so
-    # y_n = y_0 - n(n+1)/64 exactly in f32 (Verlet from rest, src/particle.rs:20-25)
+    # g = -8, dt = 1/16: acc*dt*dt = -1/32 exactly, so Verlet from rest gives
+    # y_n = -n(n+1)/64 exactly (src/particle.rs:20-25, src/solver.rs:130-141)
     o = solver(g=(0.0, -8.0))
     o.add_particle((0.0, 0.0))
     for n in range(1, 101):
         o.update(1.0 / 16.0)
         assert o.get_particles()[0, 1] == -n * (n + 1) / 64.0
-    # a term dropped from pos + vel + acc*dt*dt, or acc*dt instead of acc*dt*dt,
-    # breaks this sequence at n = 1 or 2
+        assert o.get_particles()[0, 3] == -(n - 1) * n / 64.0
 
 
 def test_substeps_split_dt():
     # EXT1: delta = dt * (1/n) (src/solver.rs:107-108); 4 substeps of 1/16 under
-    # g = -8 from rest: y = -(1/256)*8 * (1+2+3+4) = -10/32... compute exactly
+    # g = -8: each adds -1/32, y_4 = -(1+2+3+4)/32
     o = solver(g=(0.0, -8.0))
     o.set_sub_steps(4)
     o.add_particle((0.0, 0.0))
     o.update(0.25)
-    # each substep: acc*dt*dt = -8/256 = -1/32; y_4 = -(4*5/2)/32 = -10/32
     assert o.get_particles()[0, 1] == -10.0 / 32.0
+    # 3 substeps: delta = dt * fl(1/3), not dt / 3 (they differ for dt = 0.3)
+    o3 = solver(g=(0.0, -8.0))
+    o3.set_sub_steps(3)
+    o3.add_particle((0.0, 0.0))
+    o3.update(0.3)
+    d = f32(0.3) * (f32(1.0) / f32(3.0))
+    assert d != f32(0.3) / f32(3.0)
+    step = f32(f32(-8.0) * d) * d
+    y1 = f32(0.0) + step
+    y2 = f32(y1 + y1) + step
+    y3 = f32(y2 + f32(y2 - y1)) + step
+    assert exact(o3.get_particles()[0, 1], y3)
 
 
-def test_bounds_reflect_velocity_exactly():
-    # particle moving left through x = 0 (src/particle.rs:27-46)
-    o = solver(bounds=((0.0, 0.0), (100.0, 100.0)))
-    o.add_particle((-1.0, 50.0), prev=(-0.5, 50.0))
-    o.update(1.0)
-    p = o.get_particles()[0]
-    # bounds: v = prev - pos = 0.5 -> prev = 0 - 0.5, pos = 0; then integrate:
-    # vel = pos - prev = 0.5, prev = 0, pos = 0.5
-    assert (p[0], p[2]) == (0.5, 0.0)
+def test_verlet_adds_velocity_before_acceleration():
+    # pos + vel + acc*dt*dt is (pos + vel) + (acc*dt)*dt (src/particle.rs:23).
+    # pos = 1, vel = 2^-24, acc*dt*dt = 2^-24: 1 + 2^-24 is a tie and rounds to
+    # even (1), and 1 + 2^-24 again gives 1.  Adding vel + acc first would give
+    # 1 + 2^-23, which is representable.
+    pos, prev = (1.0, 1.0), (1.0 - 2.0 ** -24, 1.0)
+    p, q, a = law_particle_update(pos, prev, (2.0 ** -20, 0.0), 0.25)
+    assert exact(p, (1.0, 1.0)) and exact(q, pos) and exact(a, (0.0, 0.0))
+    # acc*dt*dt is (acc*dt)*dt: acc = 3, dt = fl(1/3): fl(3*dt) = 1 exactly, so
+    # the term is fl(1 * dt) = dt; acc*(dt*dt) would round dt*dt first
+    dt = f32(1.0) / f32(3.0)
+    p, _, _ = law_particle_update((0.0, 0.0), (0.0, 0.0), (3.0, 0.0), dt)
+    assert f32(3.0) * dt == f32(1.0)
+    assert exact(p[0], dt) and f32(3.0) * f32(dt * dt) != dt
 
 
-def test_circle_bounds_use_radius():
-    o = solver(bounds=((0.0, 0.0), (10.0, 10.0)))
-    o.add_circle((9.5, 5.0), 1.0, prev_pos=(9.5, 5.0))
-    o.update(1.0)
-    c = o.get_circles()[0]
-    # hi - r = 9 -> pos 9, prev = 9 - (9.5 - 9.5 ... ) = 9 - 0 -> then integrate
-    # v = prev - pos = 0 at clamp time: prev = 9, pos = 9; Verlet keeps 9
-    assert (c[0], c[2]) == (9.0, 9.0)
+# ---------------------------------------------------------------- bounds
+
+# Particle::solve_bounds (src/particle.rs:27-46): per axis, the low bound is
+# checked first and the high bound only if the low one did not fire; the
+# position is set to the bound B and prev to B - (prev - pos).  Bounds
+# (1.5, -2.25) + (8, 4): x in [1.5, 9.5], y in [-2.25, 1.75]; every value below
+# is a multiple of 1/8, so each result is exact.
+@pytest.mark.parametrize("pos,prev,want_pos,want_prev", [
+    ((0.5, 0.0), (1.25, 0.0), (1.5, 0.0), (0.75, 0.0)),          # x low: v = 0.75 -> prev = 1.5 - 0.75
+    ((10.5, 0.0), (9.75, 0.0), (9.5, 0.0), (10.25, 0.0)),        # x high: v = -0.75 -> prev = 9.5 + 0.75
+    ((2.0, -3.5), (2.0, -2.625), (2.0, -2.25), (2.0, -3.125)),   # y low: v = 0.875
+    ((2.0, 3.0), (2.0, 2.0), (2.0, 1.75), (2.0, 2.75)),          # y high: v = -1 -> prev = 1.75 + 1
+])
+def test_particle_bounds_each_branch(pos, prev, want_pos, want_prev):
+    p, q = law_bounds(pos, prev, (1.5, -2.25), (8.0, 4.0))
+    assert exact(p, want_pos) and exact(q, want_prev)
 
 
-def test_circle_pair_known_answer_and_centroid():
-    # equal radii 1 at distance 1.5: overlap 0.5, each moves 0.25 (src/circle.rs:32-45)
-    o = solver()
-    o.add_circle((0.0, 0.0), 1.0)
-    o.add_circle((1.5, 0.0), 1.0)
-    o.update(0.0)
-    c = o.get_circles()
-    # after update, prev_pos holds the positions collisions left (integrate runs last)
-    assert c[0, 2] == -0.25 and c[1, 2] == 1.75
-    # unequal radii: r^2-weighted centroid preserved, final distance r1 + r2
+def test_particle_bounds_reflect_velocity_exactly():
+    # all four branches with non-zero velocity: the clamped component of
+    # (pos - prev) comes out exactly negated, the other is untouched
+    bp, bs = (1.5, -2.25), (8.0, 4.0)
+    cases = [((0.5, 0.5), (0.875, 0.25), (1.5, 0.5)), ((11.0, -1.0), (10.5, -1.125), (9.5, -1.0)),
+             ((3.0, -4.0), (2.75, -3.5), (3.0, -2.25)), ((3.0, 3.0), (3.125, 2.0), (3.0, 1.75)),
+             ((0.0, -5.0), (0.5, -4.0), (1.5, -2.25)), ((12.0, 6.0), (11.0, 5.5), (9.5, 1.75))]
+    for pos, prev, want in cases:
+        p, q = law_bounds(pos, prev, bp, bs)
+        assert exact(p, want), (pos, p)
+        v_before = np.subtract(pos, prev)
+        v_after = np.subtract(p, q)
+        for ax in range(2):
+            if want[ax] != pos[ax]:
+                assert v_after[ax] == -v_before[ax]
+            else:
+                assert v_after[ax] == v_before[ax] and q[ax] == prev[ax]
+
+
+def test_particle_bounds_high_branch_association():
+    # prev.x = bounds.pos.x + bounds.size.x - vel_x is (lo + size) - v
+    # (src/particle.rs:34).  lo = 0.1, size = 8, pos 9, prev 9.25 (v = 0.25):
+    # fl(fl(0.1 + 8) - 0.25) = 7.8500004, while 0.1 + (8 - 0.25) = 7.85
+    p, q = law_bounds((9.0, 0.5), (9.25, 0.5), (0.1, 0.0), (8.0, 10.0))
+    hi = f32(f32(0.1) + f32(8.0))
+    assert exact(p, (hi, 0.5))
+    assert exact(q, (hi - f32(0.25), 0.5)) and q[0] == f32(7.8500004)
+    assert f32(f32(0.1) + f32(f32(8.0) - f32(0.25))) != q[0]
+    # the same on y (src/particle.rs:43)
+    p, q = law_bounds((0.5, 9.0), (0.5, 9.25), (0.0, 0.1), (10.0, 8.0))
+    assert exact(p, (0.5, hi)) and exact(q, (0.5, hi - f32(0.25)))
+
+
+@pytest.mark.parametrize("pos,prev,want_pos,want_prev", [
+    ((0.5, 0.0), (0.75, 0.0), (1.75, 0.0), (1.5, 0.0)),       # x low: bound lo + r = 1.75, v = 0.25
+    ((9.5, 0.0), (9.25, 0.0), (9.25, 0.0), (9.5, 0.0)),       # x high: (lo + size) - r = 9.25, v = -0.25
+    ((3.0, -3.0), (3.0, -2.75), (3.0, -2.0), (3.0, -2.25)),   # y low: -2.25 + 0.25 = -2, v = 0.25
+    ((3.0, 2.0), (3.0, 1.875), (3.0, 1.5), (3.0, 1.625)),     # y high: (-2.25 + 4) - 0.25 = 1.5, v = -0.125
+])
+def test_circle_bounds_each_branch(pos, prev, want_pos, want_prev):
+    # Circle::solve_bounds (src/circle.rs:120-139): radius-shifted bounds,
+    # prev = bound - (prev - pos), which negates the velocity component
+    p, q = law_bounds(pos, prev, (1.5, -2.25), (8.0, 4.0), radius=0.25)
+    assert exact(p, want_pos) and exact(q, want_prev)
+    v_before, v_after = np.subtract(pos, prev), np.subtract(p, q)
+    assert np.any(v_after == -v_before) and np.all(np.abs(v_after) == np.abs(v_before))
+
+
+def test_circle_bounds_association():
+    # prev.x = bounds.pos.x + self.radius - vel_x is (lo + r) - v
+    # (src/circle.rs:123): lo = 1.5, r = 0.2, pos 0.5, prev 1.25 (v = 0.75):
+    # fl(fl(1.5 + 0.2) - 0.75) = 0.95000005, while 1.5 + (0.2 - 0.75) = 0.95
+    p, q = law_bounds((0.5, 50.0), (1.25, 50.0), (1.5, 0.0), (100.0, 100.0), radius=0.2)
+    b = f32(f32(1.5) + f32(0.2))
+    assert exact(p, (b, 50.0)) and exact(q, (b - f32(0.75), 50.0)) and q[0] == f32(0.95000005)
+    assert f32(f32(1.5) + f32(f32(0.2) - f32(0.75))) != q[0]
+    # the high side is ((lo + size) - r) - v (src/circle.rs:127): lo = 0.1,
+    # size = 8, r = 0.3, pos 8.8, prev 9.05 (v = 0.25) -> 7.55, while
+    # lo + (size - (r + v)) = 7.5499997
+    p, q = law_bounds((8.8, 50.0), (9.05, 50.0), (0.1, 0.0), (8.0, 100.0), radius=0.3)
+    hi = f32(f32(f32(0.1) + f32(8.0)) - f32(0.3))
+    v = f32(9.05) - f32(8.8)
+    assert exact(p, (hi, 50.0)) and exact(q, (hi - v, 50.0))
+    assert f32(f32(0.1) + f32(f32(8.0) - f32(f32(0.3) + v))) != q[0]
+    # and y, both sides (src/circle.rs:130-138)
+    p, q = law_bounds((50.0, 0.5), (50.0, 1.25), (0.0, 1.5), (100.0, 100.0), radius=0.2)
+    assert exact(p, (50.0, b)) and exact(q, (50.0, b - f32(0.75)))
+    p, q = law_bounds((50.0, 8.8), (50.0, 9.05), (0.0, 0.1), (100.0, 8.0), radius=0.3)
+    assert exact(p, (50.0, hi)) and exact(q, (50.0, hi - v))
+
+
+# ---------------------------------------------------------------- links
+
+def test_single_link_known_answer():
+    # one link (0,0)-(2,0), target 1: each end moves 0.5 inward, exactly
+    # (src/link.rs:78-87: normal * (dist - target) * 0.5)
+    a, b = law_particle_link((0.0, 0.0), (2.0, 0.0), 1.0)
+    assert exact(a, (0.5, 0.0)) and exact(b, (1.5, 0.0))
+    # a 3-4-5 link with target 0: both ends meet at the midpoint (1.5, 2)
+    a, b = law_particle_link((0.0, 0.0), (3.0, 4.0), 0.0)
+    assert exact(a, (1.5, 2.0)) and exact(b, (1.5, 2.0))
+    # stretched: target 15 from length 5 pushes each end 5 outward along the line
+    a, b = law_particle_link((0.0, 0.0), (3.0, 4.0), 15.0)
+    assert abs(a[0] + 3.0) < 4e-7 and abs(a[1] + 4.0) < 4e-7
+    assert abs(b[0] - 6.0) < 8e-7 and abs(b[1] - 8.0) < 8e-7
+
+
+def test_link_normal_is_a_division():
+    # normalize() divides each component by the length (nalgebra's unscale);
+    # for d = (-1, -24)/... pick |d| = 5 with components 3 and 4: 3/5 and 4/5
+    # are correctly rounded quotients.  A reciprocal-multiply 3 * fl(1/5)
+    # differs for some inputs; here every link of the sweep (length 5, 13, 25,
+    # 17, 41) must move its ends by exactly fl(fl(c/L) * e) * 0.5.
+    for (x, y, L) in [(3, 4, 5), (5, 12, 13), (7, 24, 25), (8, 15, 17), (9, 40, 41), (20, 21, 29)]:
+        for t in (1.0, 0.5, 2.75):
+            a, b = law_particle_link((0.0, 0.0), (float(x), float(y)), t)
+            e = f32(L) - f32(t)
+            nx, ny = f32(-x) / f32(L), f32(-y) / f32(L)
+            assert exact(a, (f32(0.0) - f32(f32(nx * e) * f32(0.5)), f32(0.0) - f32(f32(ny * e) * f32(0.5))))
+            assert exact(b, (f32(x) + f32(f32(nx * e) * f32(0.5)), f32(y) + f32(f32(ny * e) * f32(0.5))))
+    # at least one of these inputs separates division from reciprocal multiply
+    assert any(f32(c) / f32(L) != f32(c) * (f32(1.0) / f32(L))
+               for (x, y, L) in [(3, 4, 5), (5, 12, 13), (7, 24, 25), (8, 15, 17), (9, 40, 41), (20, 21, 29)]
+               for c in (x, y))
+
+
+def test_link_half_applied_after_the_product():
+    # normal * (dist - target) * 0.5 multiplies by 0.5 last (src/link.rs:85).
+    # Only in the subnormal range can this differ from normal * (e * 0.5): a
+    # link almost along y whose x components are ~1e-37, so normal.x * e is
+    # subnormal and rounds twice when halved afterwards.
+    a, b = (f32(5.492344e-38), f32(1.0717883)), (0.0, 0.0)
+    t = f32(1.0708414)
+    ga, gb = law_particle_link(a, b, t)
+    d = np.sqrt(f32(a[0] * a[0] + a[1] * a[1]))
+    nx, e = a[0] / d, d - t
+    assert exact(ga[0], a[0] - f32(nx * e) * f32(0.5))
+    assert f32(a[0] - f32(nx * e) * f32(0.5)) != f32(a[0] - nx * f32(e * f32(0.5)))
+
+
+def test_link_preserves_midpoint_and_reaches_target():
+    # in f64, an isolated link keeps its midpoint and ends at its target length
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a, b = rng.uniform(-10, 10, 2), rng.uniform(-10, 10, 2)
+        t = float(rng.uniform(0.1, 5))
+        ga, gb = law_particle_link(a, b, t, bits=64)
+        assert np.allclose((ga + gb) / 2, (a + b) / 2, atol=1e-12)
+        assert abs(np.linalg.norm(ga - gb) - t) < 1e-9
+
+
+def test_pinned_link_moves_only_free_end():
+    # EXT3: a pinned end stays, the free end takes the whole correction
+    a, b = law_particle_link((0.0, 0.0), (3.0, 4.0), 1.0, pinned_a=True)
+    assert exact(a, (0.0, 0.0))
+    # free end lands on the target circle around the pinned end: (3,4) - (0.6,0.8)*4
+    e = f32(5.0) - f32(1.0)
+    assert exact(b, (f32(3.0) + f32(f32(f32(-3.0) / f32(5.0)) * e), f32(4.0) + f32(f32(f32(-4.0) / f32(5.0)) * e)))
+    assert abs(b[0] - 0.6) < 1e-6 and abs(b[1] - 0.8) < 1e-6
+    a, b = law_particle_link((0.0, 0.0), (3.0, 4.0), 1.0, pinned_b=True)
+    assert exact(b, (3.0, 4.0)) and abs(a[0] - 2.4) < 1e-6 and abs(a[1] - 3.2) < 1e-6
+    a, b = law_particle_link((0.0, 0.0), (3.0, 4.0), 1.0, pinned_a=True, pinned_b=True)
+    assert exact(a, (0.0, 0.0)) and exact(b, (3.0, 4.0))
+
+
+def test_circle_link_weights_known_answer():
+    # CircleLink::solve (src/link.rs:96-108): a moves by rb^2/(ra^2+rb^2) of the
+    # error, b by ra^2/(ra^2+rb^2).  Equal radii 2: scale = 1/8, each end gets
+    # e * (1/8) * 4 = e/2 exactly.
+    a, b = law_circle_link((0.0, 0.0), (5.0, 0.0), 2.0, 2.0, 4.0)
+    assert exact(a, (0.5, 0.0)) and exact(b, (4.5, 0.0))
+    # radii 1 and 3, e = 1, normal (-1, 0): a -= ((n * e) * scale) * 9 and
+    # b += ((n * e) * scale) * 1 with scale = fl(1/10): a.x = 0.90000004, b.x = 4.9
+    s = f32(1.0) / f32(10.0)
+    a, b = law_circle_link((0.0, 0.0), (5.0, 0.0), 1.0, 3.0, 4.0)
+    assert exact(a, (f32(0.0) - f32(f32(f32(-1.0) * s) * f32(9.0)), 0.0))
+    assert exact(b, (f32(5.0) + f32(f32(-1.0) * s) * f32(1.0), 0.0))
+    assert abs(a[0] - 0.9) < 1e-6 and abs(b[0] - 4.9) < 1e-6
+    # the grouping is ((n * e) * scale) * rb^2: with e = 7 it gives 6.2999997,
+    # e * (scale * rb^2) would give 6.3
+    a, b = law_circle_link((0.0, 0.0), (11.0, 0.0), 1.0, 3.0, 4.0)
+    want = f32(0.0) - f32(f32(f32(-1.0) * f32(7.0)) * s) * f32(9.0)
+    assert exact(a, (want, 0.0)) and want == f32(6.2999997)
+    assert f32(0.0) - f32(-1.0) * f32(f32(7.0) * f32(s * f32(9.0))) != want
+
+
+def test_circle_link_weighted_centroid():
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        a, b = rng.uniform(-10, 10, 2), rng.uniform(-10, 10, 2)
+        ra, rb = rng.uniform(0.2, 3, 2)
+        t = float(rng.uniform(0.1, 5))
+        ga, gb = law_circle_link(a, b, ra, rb, t, bits=64)
+        wa, wb = ra * ra, rb * rb
+        assert np.allclose((wa * ga + wb * gb) / (wa + wb), (wa * a + wb * b) / (wa + wb), atol=1e-9)
+        assert abs(np.linalg.norm(ga - gb) - t) < 1e-9
+
+
+# ---------------------------------------------------------------- circle pairs
+
+def test_circle_pair_known_answer():
+    # equal radii 1 at distance 1.5: overlap 0.5, scale 1/2, each moves
+    # 1 * 0.5 * 0.5 * 1 = 0.25 apart (src/circle.rs:141-154)
+    hit, a, b = law_circle_pair((0.0, 0.0), (1.5, 0.0), 1.0, 1.0)
+    assert hit and exact(a, (-0.25, 0.0)) and exact(b, (1.75, 0.0))
+    # along a 3-4-5 diagonal, radii 4 and 4, distance 5: overlap 3, each moves 1.5
+    hit, a, b = law_circle_pair((3.0, 4.0), (0.0, 0.0), 4.0, 4.0)
+    assert hit
+    n = (f32(3.0) / f32(5.0), f32(4.0) / f32(5.0))
+    mv = [f32(f32(f32(c * f32(1.0 / 32.0)) * f32(3.0)) * f32(16.0)) for c in n]
+    assert exact(a, (f32(3.0) + mv[0], f32(4.0) + mv[1])) and exact(b, (f32(0.0) - mv[0], f32(0.0) - mv[1]))
+
+
+def test_circle_pair_touching_is_no_contact():
+    # dist_sqr < radius_sum^2 is strict: circles exactly touching do not move
+    hit, a, b = law_circle_pair((0.0, 0.0), (2.0, 0.0), 1.0, 1.0)
+    assert not hit and exact(a, (0, 0)) and exact(b, (2, 0))
+    hit, a, b = law_circle_pair((0.0, 0.0), (3.0, 4.0), 2.0, 3.0)
+    assert not hit and exact(b, (3, 4))
+
+
+def test_circle_pair_weights_go_to_the_other_radius():
+    # self moves by circle_rad_sqr, circle by self_rad_sqr: radii 1 and 3 at
+    # distance 2: overlap 2, scale fl(1/10); self moves fl(fl(fl(-1*s)*2)*9)
+    hit, a, b = law_circle_pair((0.0, 0.0), (2.0, 0.0), 1.0, 3.0)
+    s = f32(1.0) / f32(10.0)
+    assert hit
+    assert exact(a[0], f32(0.0) + f32(f32(f32(f32(-1.0) * s) * f32(2.0)) * f32(9.0)))
+    assert exact(b[0], f32(2.0) - f32(f32(f32(f32(-1.0) * s) * f32(2.0)) * f32(1.0)))
+    assert abs(a[0] + 1.8) < 1e-6 and abs(b[0] - 2.2) < 1e-6
+
+
+def test_circle_pair_centroid_and_distance():
     rng = np.random.default_rng(4)
     for _ in range(100):
         r1, r2 = rng.uniform(0.5, 3, 2)
         p1 = rng.uniform(-5, 5, 2)
         p2 = p1 + rng.uniform(-1, 1, 2)
-        o = solver(64)
-        o.add_circle(p1, r1)
-        o.add_circle(p2, r2)
-        o.update(0.0)
-        c = o.get_circles()
-        w1, w2 = np.float32(r1) ** 2, np.float32(r2) ** 2
-        before = (w1 * np.float32(p1).astype(np.float64) + w2 * np.float32(p2)) / (w1 + w2)
-        after = (w1 * c[0, 2:4] + w2 * c[1, 2:4]) / (w1 + w2)
-        assert np.allclose(before, after, atol=1e-9)
-        assert abs(np.linalg.norm(c[0, 2:4] - c[1, 2:4]) - (float(np.float32(r1)) + float(np.float32(r2)))) < 1e-9
+        hit, a, b = law_circle_pair(p1, p2, r1, r2, bits=64)
+        assert hit
+        w1, w2 = r1 * r1, r2 * r2
+        assert np.allclose((w1 * a + w2 * b) / (w1 + w2), (w1 * p1 + w2 * p2) / (w1 + w2), atol=1e-9)
+        assert abs(np.linalg.norm(a - b) - (r1 + r2)) < 1e-9
 
 
-def test_circle_link_weights():
-    # CircleLink (src/link.rs:36-48): a moves by rb^2/(ra^2+rb^2) of the error
+def test_circle_pairs_run_in_lexicographic_order():
+    # three unit circles on a line at 0, 1.5, 3 (src/solver.rs:167-177 runs
+    # (0,1), (0,2), (1,2) with live positions):
+    #   (0,1): overlap 0.5 -> 0 to -0.25, 1 to 1.75
+    #   (0,2): distance 3.25, no contact
+    #   (1,2): distance 1.25, overlap 0.75 -> 1 to 1.375, 2 to 3.375
     o = solver()
-    o.add_circle((0.0, 0.0), 1.0)
-    o.add_circle((5.0, 0.0), 3.0)
-    o.add_circle_link(0, 1, 4.0)
+    for x in (0.0, 1.5, 3.0):
+        o.add_circle((x, 0.0), 1.0)
     o.update(0.0)
     c = o.get_circles()
-    # error 1; a moves +0.9, b moves -0.1 (weights 9/10, 1/10)
-    assert abs(c[0, 2] - 0.9) < 1e-6 and abs(c[1, 2] - 4.9) < 1e-6
+    assert exact(c[:, 2], (-0.25, 1.375, 3.375))   # prev holds the post-collision positions
+    # any other order gives other positions: e.g. (1,2) first
+    o2 = solver()
+    for x in (0.0, 1.5, 3.0):
+        o2.add_circle((x, 0.0), 1.0)
+    o2.add_circle((100.0, 0.0), 1.0)
+    o2.update(0.0)
+    assert exact(o2.get_circles()[:3, 2], (-0.25, 1.375, 3.375))
 
+
+# ---------------------------------------------------------------- segments
+
+@pytest.mark.parametrize("p1,p2,p3,p4,want", [
+    ((0, 0), (2, 0), (1, -1), (1, 1), (1.0, 0.0)),        # crossing at s = t = 1/2
+    ((0, 0), (4, 4), (0, 4), (4, 0), (2.0, 2.0)),         # diagonals
+    ((0, 0), (2, 0), (0, -1), (0, 1), (0.0, 0.0)),        # t = 0: the first segment's start
+    ((0, 0), (2, 0), (2, 1), (2, -3), (2.0, 0.0)),        # t = 1: the first segment's end
+    ((0, 0), (2, 0), (1, 0), (1, 5), (1.0, 0.0)),         # s = 0: the second segment starts on the first
+    ((0, 0), (2, 0), (1, 5), (1, 0), (1.0, 0.0)),         # s = 1: it ends on it
+    ((0, 0), (2, 0), (0, 0), (0, 3), (0.0, 0.0)),         # s = 0 and t = 0
+    ((0, 0), (2, 0), (3, -1), (3, 1), None),              # t = 1.5 > 1
+    ((0, 0), (2, 0), (1, 0.5), (1, 2), None),             # s < 0
+    ((0, 0), (2, 0), (1, -2), (1, -0.5), None),           # s > 1
+    ((0, 0), (2, 0), (0, 1), (2, 1), None),               # parallel: division by 0 -> inf, no hit
+    ((0, 0), (2, 0), (1, 0), (3, 0), None),               # collinear: 0/0 = NaN, comparisons false
+    ((-1, 0), (0, 0), (-0.5, -1), (-0.5, 1), (-0.5, 0.0)),
+])
+def test_line_intersection_cases(p1, p2, p3, p4, want):
+    # common::line_intersection (src/common.rs:4-26): s along p3-p4, t along
+    # p1-p2, both in [0, 1] inclusive; the point is p1 + t * (p2 - p1)
+    got = law_line_intersection(p1, p2, p3, p4)
+    if want is None:
+        assert got is None
+    else:
+        assert got is not None and exact(got, want)
+
+
+def test_line_intersection_point_is_on_the_first_segment():
+    # the point is p1 + t * s1: for p1 = (0, 1), p2 = (4, 1) and a crossing at
+    # x = 3 the point must be (3, 1), not p3 + s * s2's rounding
+    got = law_line_intersection((0, 1), (4, 1), (3, -5), (3, 3))
+    assert exact(got, (3.0, 1.0))
+    got = law_line_intersection((1, 0), (1, 8), (-1, 6), (5, 6))
+    assert exact(got, (1.0, 6.0))
+
+
+# ---------------------------------------------------------------- polygons
+
+def test_polygon_edge_push_known_answer():
+    # A point of another polygon just inside edge (0,0)-(2,0) of a static
+    # square with centre (1,1); the other polygon's centre is below the edge.
+    # src/polygon.rs:164-216: the segment point -> centre crosses the edge at
+    # (1,0) with t = 1/2; the centre projects onto the line at (1,0), so the
+    # inward normal is (0,1); each edge end gets influence 1/2 of the line
+    # displacement 2/3 * (intersection - point) . n = 2/3 * (-0.5) -> the ends
+    # move to y = 1/6 (fl of 0.5 * fl(fl(-0.5/3) * 2)), and the point moves to
+    # the edge along -n: (1, 0).
+    o = solver()
+    sq = np.array([[0, 0, 0, 0, 0, 0], [2, 0, 2, 0, 0, 0], [2, 2, 2, 2, 0, 0], [0, 2, 0, 2, 0, 0]], np.float32)
+    o.add_polygon(sq, ([0, 1, 2, 0], [1, 2, 3, 3], [2, 2, 2, 2]), True, (1.0, 1.0))
+    tri = np.array([[1, 0.5, 1, 0.5, 0, 0], [1.3, -3.25, 1.3, -3.25, 0, 0], [0.7, -3.25, 0.7, -3.25, 0, 0]],
+                   np.float32)
+
+    def d(i, j):
+        return float(np.float32(np.hypot(*(tri[i, :2] - tri[j, :2]))))
+    o.add_polygon(tri, ([0, 1, 0], [1, 2, 2], [d(0, 1), d(1, 2), d(0, 2)]), True, (1.0, -2.0))
+    o.update(0.0)
+    sqo = o.get_polygon(0)["particles"]
+    tro = o.get_polygon(1)["particles"]
+    third = f32(f32(-0.5) / f32(3.0))
+    disp = f32(f32(0.5) * f32(third * f32(2.0)))
+    assert exact(sqo[0, :2], (0.0, f32(0.0) - disp)) and exact(sqo[1, :2], (2.0, f32(0.0) - disp))
+    assert abs(sqo[0, 1] - 1 / 6) < 1e-7
+    assert exact(tro[0, :2], (1.0, 0.0))
+    # the triangle's edges then hit nothing of the square; its other points stay
+    assert exact(tro[1:, :2], tri[1:, :2])
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 7, 8, 9, 12, 13])
+def test_polygon_circle_topology_and_chords(n):
+    # Polygon::circle (src/polygon.rs:17-82): point i at angle ~ 2*pi*i/n on the
+    # circle; links (i, (i + 2n/3) mod n) then (i, (i + n/3) mod n) per i,
+    # integer division, each swapped so a < b; targets are the current lengths,
+    # i.e. chords 2 R sin(pi k / n) for the index offset k
+    R, c = 5.0, (10.0, 20.0)
+    o = solver()
+    o.add_polygon_circle(R, c, n, False)
+    g = o.get_polygon(0)
+    pts = g["particles"][:, :2]
+    ang = np.arctan2(pts[:, 1] - c[1], pts[:, 0] - c[0]) % (2 * np.pi)
+    want_ang = (2 * np.pi * np.arange(n) / n) % (2 * np.pi)
+    assert np.allclose(np.minimum(abs(ang - want_ang), 2 * np.pi - abs(ang - want_ang)), 0, atol=1e-5)
+    assert np.allclose(np.hypot(pts[:, 0] - c[0], pts[:, 1] - c[1]), R, atol=2e-6 * R)
+    assert abs(g["center"][0] - c[0]) < 1e-5 and abs(g["center"][1] - c[1]) < 1e-5
+    a, b, t = g["links"]
+    want = []
+    for i in range(n):
+        for k in (2 * n // 3, n // 3):
+            x, y = i, (i + k) % n
+            want.append((min(x, y), max(x, y), k))
+    assert list(zip(a.tolist(), b.tolist())) == [(x, y) for x, y, _ in want]
+    for (x, y, k), tt in zip(want, t):
+        assert abs(tt - 2 * R * math.sin(math.pi * k / n)) < 4e-6 * R
+        dd = np.float32(pts[x]) - np.float32(pts[y])
+        assert exact(tt, np.sqrt(f32(dd[0] * dd[0] + dd[1] * dd[1])))
+
+
+@pytest.mark.parametrize("n", [3, 4, 6, 9])
+def test_polygon_new_topology(n):
+    # Polygon::new (src/polygon.rs:84-123): edge links (i, (i+1) mod n) swapped
+    # so a < b (the last is (0, n-1)); centre = sum / n in index order
+    pts = np.array([[0.5 * i, (i * i) % 7 * 0.25] for i in range(n)], np.float32)
+    o = solver()
+    o.add_polygon_new(pts, True)
+    g = o.get_polygon(0)
+    a, b, t = g["links"]
+    assert list(zip(a.tolist(), b.tolist())) == [(i, i + 1) for i in range(n - 1)] + [(0, n - 1)]
+    cx = f32(0.0)
+    cy = f32(0.0)
+    for p in pts:
+        cx, cy = cx + p[0], cy + p[1]
+    assert exact(g["center"], (cx / f32(n), cy / f32(n)))
+    for (x, y), tt in zip(zip(a, b), t):
+        dd = pts[x] - pts[y]
+        assert exact(tt, np.sqrt(f32(dd[0] * dd[0] + dd[1] * dd[1])))
+
+
+def test_static_polygon_is_not_integrated_but_is_linked_and_bounded():
+    # Polygon::update returns early when static (src/polygon.rs:125-128), but
+    # its links are solved (src/solver.rs:150-152) and bounds apply (163)
+    o = solver(g=(0.0, -8.0), bounds=((0.0, 0.0), (10.0, 10.0)))
+    pts = np.array([[1, 1, 1, 1, 0, 0], [4, 1, 4, 1, 0, 0], [12, 5, 11, 5, 0, 0]], np.float32)
+    o.add_polygon(pts, ([0], [1], [2.0]), True, (0.0, 0.0))
+    o.update(1.0 / 16.0)
+    g = o.get_polygon(0)["particles"]
+    # link 0-1 (length 3, target 2): each end 0.5 inward; point 2 clamped to x = 10
+    assert exact(g[:, :2], ((1.5, 1.0), (3.5, 1.0), (10.0, 5.0)))
+    # prev = 10 - (11 - 12) = 11: reflected; no integration, so prev stays and
+    # acc keeps the gravity it accumulated (src/polygon.rs:126-128, 225-229)
+    assert exact(g[:, 2:4], ((1.0, 1.0), (4.0, 1.0), (11.0, 5.0)))
+    assert exact(g[:, 5], (-8.0, -8.0, -8.0))
+    # a free copy is integrated: gravity moves it by -8/256
+    o2 = solver(g=(0.0, -8.0), bounds=((0.0, 0.0), (10.0, 10.0)))
+    o2.add_polygon(pts[:2], ([0], [1], [2.0]), False, (0.0, 0.0))
+    o2.update(1.0 / 16.0)
+    assert exact(o2.get_polygon(0)["particles"][:, 1], (1.0 - 1 / 32.0, 1.0 - 1 / 32.0))
+
+
+def test_polygon_centre_is_a_mean_in_index_order():
+    # calc_center (src/polygon.rs:231-237): sum in index order, then / n;
+    # after a free polygon's update the centre is that of the pre-integration
+    # positions (Polygon::update computes it before integrating, :130)
+    o = solver(bounds=((-1e9, -1e9), (2e9, 2e9)))
+    pts = np.array([[1e8, 0, 1e8, 0, 0, 0], [1, 0, 1, 0, 0, 0], [-1e8, 0, -1e8, 0, 0, 0]], np.float32)
+    o.add_polygon(pts, ([], [], []), False, (0.0, 0.0))
+    o.update(0.0)
+    # (1e8 + 1) rounds to 1e8, - 1e8 = 0 -> centre 0; another order would give 1/3
+    assert exact(o.get_polygon(0)["center"], (0.0, 0.0))
+
+
+# ---------------------------------------------------------------- extensions
 
 def test_static_circle_collider():
     o = solver()
     o.add_static_circle((0.0, 0.0), 2.0)
     o.add_particle((1.0, 0.0))
+    o.add_circle((0.0, -2.5), 1.0)
     o.update(0.0)
-    assert o.get_particles()[0, 2] == 2.0
+    assert exact(o.get_particles()[0, 2:4], (2.0, 0.0))
+    assert exact(o.get_circles()[0, 2:4], (0.0, -3.0))
 
 
 def test_static_polygon_collider_min_penetration():
-    # unit square [0,4]^2; a point at (3.5, 1) leaves through x = 4 (least depth)
+    # square [0,4]^2; a point at (3.5, 1) leaves through x = 4 (least depth)
     o = solver()
     o.add_static_polygon([(0, 0), (4, 0), (4, 4), (0, 4)])
     o.add_particle((3.5, 1.0))
@@ -189,46 +558,23 @@ def test_static_polygon_collider_min_penetration():
     assert tuple(o2.get_particles()[0, 2:4]) == (4.0, 1.0)
 
 
-def test_polygon_edge_push_known_answer():
-    # A point of another polygon inside edge (0,0)-(2,0) of a square, with the
-    # other polygon's centre below the edge: the segment point->centre crosses
-    # the edge at (1,0) (s = 1/3, t = 1/2), the edge ends move 1/6 inward along
-    # the inward normal (0,1) and the point is projected onto the edge at (1,0)
-    # (src/polygon.rs:164-216, src/common.rs:4-26).
-    o = solver()
-    sq = np.array([[0, 0, 0, 0, 0, 0], [2, 0, 2, 0, 0, 0], [2, 2, 2, 2, 0, 0], [0, 2, 0, 2, 0, 0]], np.float32)
-    o.add_polygon(sq, ([0, 1, 2, 0], [1, 2, 3, 3], [2, 2, 2, 2]), True, (1.0, 1.0))
-    tri = np.array([[1, 0.5, 1, 0.5, 0, 0], [1.3, -3.25, 1.3, -3.25, 0, 0], [0.7, -3.25, 0.7, -3.25, 0, 0]],
-                   np.float32)
-    # links with their current lengths so link solving does not move anything
-    def d(i, j):
-        return float(np.float32(np.hypot(*(tri[i, :2] - tri[j, :2]))))
-    o.add_polygon(tri, ([0, 1, 0], [1, 2, 2], [d(0, 1), d(1, 2), d(0, 2)]), True, (1.0, -2.0))
-    o.update(0.0)
-    sqo = o.get_polygon(0)["particles"]
-    tro = o.get_polygon(1)["particles"]
-    assert abs(sqo[0, 1] - 1 / 6) < 1e-6 and abs(sqo[1, 1] - 1 / 6) < 1e-6
-    assert sqo[0, 0] == 0 and sqo[1, 0] == 2
-    assert abs(tro[0, 0] - 1.0) < 1e-6 and abs(tro[0, 1]) < 1e-6
-
-
-def test_polygon_ctor_circle_geometry():
-    # Polygon::circle (src/polygon.rs:17-82): n points on the circle, links to
-    # i+2n/3 and i+n/3 with their current lengths, a < b
-    o = solver()
-    o.add_polygon_circle(5.0, (10.0, 20.0), 9, False)
-    g = o.get_polygon(0)
-    pts = g["particles"][:, :2]
-    assert np.allclose(np.hypot(pts[:, 0] - 10, pts[:, 1] - 20), 5.0, atol=1e-5)
-    assert abs(g["center"][0] - 10) < 1e-5 and abs(g["center"][1] - 20) < 1e-5
+def test_pinned_particle_never_moves():
+    # EXT3: no gravity, bounds or collider moves a pinned particle
+    o = solver(g=(0.0, -100.0), bounds=((0.0, 0.0), (1.0, 1.0)))
+    o.add_particle((5.0, 5.0))
+    o.add_static_circle((5.0, 5.0), 1.0)
+    o.pin(0)
+    for _ in range(3):
+        o.update(0.1)
+    assert exact(o.get_particles()[0, :4], (5.0, 5.0, 5.0, 5.0))
 
 
 def test_candidate_sweep_equals_all_pairs():
-    # two independent implementations of A7's lexicographic sweep must agree
+    # two independent implementations of the lexicographic sweep must agree
     # bit for bit (the candidate list is how large scenes are checked)
     for seed in range(3):
         rng = np.random.default_rng(seed)
-        a, b = solver(g=(0.0, -9.81), bounds=((0, 0), (30, 30))), None
+        a = solver(g=(0.0, -9.81), bounds=((0, 0), (30, 30)))
         b = solver(g=(0.0, -9.81), bounds=((0, 0), (30, 30)))
         b.set_candidate_mode(True, 0.05)
         for _ in range(300):
@@ -256,6 +602,17 @@ def test_order_of_link_sweep_matters():
         o.apply_particle_links_in_order(order)
         return o.get_particles()
     assert not bits_equal(run([0, 1, 2]), run([2, 1, 0]))
+
+
+def test_gravity_accumulates_once_per_substep():
+    # apply_gravity adds g once per substep (src/solver.rs:130-141); a circle
+    # with an initial acc keeps it for the first substep only
+    o = solver(g=(0.0, -8.0))
+    o.add_circle((0.0, 0.0), 0.5, acc=(0.0, -8.0))
+    o.update(1.0 / 16.0)
+    assert exact(o.get_circles()[0, 1], -2.0 / 32.0)
+    o.update(1.0 / 16.0)
+    assert exact(o.get_circles()[0, 1], -2.0 / 32.0 * 2 - 1 / 32.0)
 
 
 def test_f64_twin_close_to_f32():
