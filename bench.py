@@ -267,39 +267,46 @@ def copy_calibration(nbytes=1 << 30):
     return {"bytes": nbytes, "ms": ms.value, "copy_GBps": 2.0 * nbytes / (ms.value * 1e-3) / 1e9}
 
 
-def cpu_baseline(seconds=12.0, seed0=1000):
-    """The f32 oracle (single thread) on the batched workload's scenes, one
-    update per scene in scene order until `seconds` of CPU time elapse."""
+def _oracle_imports():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from helpers import to_oracle
-    from bendy2d_amd import Solver, workloads
-    pu, busy, n = 0, 0.0, 0
-    while busy < seconds and n < 4096:
-        s = Solver()
-        workloads.softbody(s, seed0 + n)
-        o = to_oracle(s, 32, candidate_mode=True)
-        npts = s.stats()["n_points"]
-        t0 = time.process_time()
-        o.update(DT)
-        busy += time.process_time() - t0
-        pu += npts * 8
-        n += 1
-    return {"value": pu / busy, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-            "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off), 1 thread: the first {n} scenes of the "
-                      f"4096-scene workload, 1 update (8 substeps) each, {busy:.1f} s CPU"}
+    return to_oracle
 
 
-def cpu_plain_runs():
-    """The f32 oracle (single thread) on the plain path's two small figures:
-    the 32x32 pinned cloth's whole 1,000-update run, and the first update of
-    the 262,144-circle gas box (the candidate-list form of the reference's
-    all-pairs loop, which it equals bit for bit)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from helpers import to_oracle
+def cpu_leg(seconds_scenes=12.0):
+    """The CPU baselines, run in a child process pinned to one core before it
+    loads anything (it never touches the GPU): the f32 oracle (gcc -O2
+    -ffp-contract=off, one thread), a C restatement of the reference's solver,
+    timed with process CPU time on bounded samples of each workload."""
+    to_oracle = _oracle_imports()
     from bendy2d_amd import Solver, workloads
     out = {}
+    label = "f32 oracle (C, gcc -O2 -ffp-contract=off), 1 thread"
+
+    # 4,096-scene workload: the reference's all-pairs circle loop, and the
+    # candidate-list form of it (same results bit for bit), one update each
+    for mode, secs in (("all_pairs", seconds_scenes), ("candidate_list", 0.5 * seconds_scenes)):
+        pu, busy, n = 0, 0.0, 0
+        while busy < secs and n < 4096:
+            s = Solver()
+            workloads.softbody(s, 1000 + n)
+            o = to_oracle(s, 32, candidate_mode=(mode == "candidate_list"))
+            npts = s.stats()["n_points"]
+            t0 = time.process_time()
+            o.update(DT)
+            busy += time.process_time() - t0
+            pu += npts * 8
+            n += 1
+            s.close()
+        loop = ("the reference's all-pairs circle loop (src/solver.rs:168-177)" if mode == "all_pairs"
+                else "candidate lists in place of the all-pairs loop (bit-identical results)")
+        out[f"softbody4096_{mode}"] = {
+            "value": pu / busy, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{label}, {loop}: scenes 0..{n - 1} of the 4096-scene workload, update 1 (8 substeps) "
+                      f"each, {busy:.1f} s CPU"}
+
+    # 32x32 pinned cloth: the whole 1,000-update run
     s = Solver()
     dt, steps = workloads.cloth_pinned(s)
     o = to_oracle(s, 32)
@@ -308,20 +315,99 @@ def cpu_plain_runs():
         o.update(dt)
     t = time.process_time() - t0
     out["cloth_32"] = {"value": 1024 * 8 * steps / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-                       "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off), 1 thread: the whole {steps}-update "
-                                 f"run, {t:.2f} s CPU"}
+                       "sample": f"{label}: the whole {steps}-update run, {t:.2f} s CPU"}
+    s.close()
+
+    # 1024x1024 cloth: the first updates from rest until 20 s, extrapolated
+    s = Solver()
+    dt, _ = workloads.cloth_large(s)
+    o = to_oracle(s, 32)
+    n, t = 0, 0.0
+    while t < 20.0 and n < 25:
+        t0 = time.process_time()
+        o.update(dt)
+        t += time.process_time() - t0
+        n += 1
+    N = s.stats()["n_points"]
+    out["cloth_1024"] = {"value": N * 4 * n / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+                         "sample": f"{label}: updates 1..{n} ({4 * n} substeps) from rest, {t:.1f} s CPU; "
+                                   "the 500-update run extrapolated at this rate"}
+    s.close()
+
+    # 262,144-circle gas box: update 1 with candidate lists, and a slice of the
+    # reference's all-pairs loop timed and extrapolated to its C(C-1)/2 tests
     s = Solver()
     dt, _ = workloads.gas_box(s)
+    N = s.stats()["n_points"]
     o = to_oracle(s, 32, candidate_mode=True)
-    n = s.stats()["n_points"]
     t0 = time.process_time()
     o.update(dt)
     t = time.process_time() - t0
-    out["gas_262144"] = {"value": n * 4 / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-                         "sample": f"f32 oracle (C, gcc -O2 -ffp-contract=off, candidate lists), 1 thread: "
-                                   f"update 1 (4 substeps), {t:.2f} s CPU"}
+    out["gas_262144_candidate_list"] = {
+        "value": N * 4 / t, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+        "sample": f"{label}, candidate lists in place of the all-pairs loop (bit-identical results): update 1 "
+                  f"(4 substeps), {t:.2f} s CPU"}
+    o2 = to_oracle(s, 32)
+    rows, tests, t = 0, 0, 0.0
+    while t < 5.0 and rows < N:
+        t0 = time.process_time()
+        tests += o2.all_pairs_rows(rows, rows + 256)
+        t += time.process_time() - t0
+        rows += 256
+    rows = min(rows, N)
+    per_substep = (N * (N - 1) / 2) / (tests / t)
+    out["gas_262144_all_pairs"] = {
+        "value": N / per_substep, "unit": "particle-updates/s", "cores": 1, "kind": "port",
+        "sample": f"{label}, the reference's all-pairs circle loop (src/solver.rs:168-177): rows 0..{rows - 1} of "
+                  f"the first substep's loop ({tests} pair tests) in {t:.2f} s CPU, extrapolated to the "
+                  f"{N * (N - 1) // 2} tests of one substep ({per_substep:.0f} s per substep)"}
     s.close()
     return out
+
+
+def host_cpu_info(core):
+    info = {"core": core, "host_cpus": os.cpu_count(), "cpu_model": "unknown", "mhz": None, "max_mhz": None}
+    try:
+        with open("/proc/cpuinfo") as f:
+            cur = None
+            for line in f:
+                if line.startswith("processor"):
+                    cur = int(line.split(":")[1])
+                elif line.startswith("model name") and info["cpu_model"] == "unknown":
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                elif line.startswith("cpu MHz") and cur == core:
+                    info["mhz"] = float(line.split(":")[1])
+    except OSError:
+        pass
+    for key, name in (("mhz_scaling", "scaling_cur_freq"), ("max_mhz", "cpuinfo_max_freq")):
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{core}/cpufreq/{name}") as f:
+                info[key] = int(f.read()) / 1000.0
+        except (OSError, ValueError):
+            pass
+    return info
+
+
+def cpu_baselines():
+    """Runs cpu_leg() in a child process pinned to one core (the last of this
+    process's affinity set) before it imports anything; returns its results
+    with the host's CPU model, core and clock."""
+    import subprocess
+    core = sorted(os.sched_getaffinity(0))[-1]
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", str(core)],
+                       capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log(f"cpu leg failed ({r.returncode}): {r.stderr[-2000:]}")
+        return None
+    data = json.loads(r.stdout.strip().splitlines()[-1])
+    host = host_cpu_info(core)
+    for v in data.values():
+        v["pinned_core"] = core
+        v["cpu_model"] = host["cpu_model"]
+        v["mhz"] = host.get("mhz_scaling") or host.get("mhz")
+        v["max_mhz"] = host.get("max_mhz")
+        v["host_cpus"] = host["host_cpus"]
+    return data
 
 
 def profiled_traffic_per_update():
@@ -335,17 +421,6 @@ def profiled_traffic_per_update():
         return float(json.loads(block)["traffic_bytes_per_update"])
     except (OSError, IndexError, KeyError, ValueError):
         return None
-
-
-def cpu_model():
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
 
 
 def main():
@@ -371,7 +446,14 @@ def main():
     ap.add_argument("--launch", type=str, default="", help="batch launch config as JSON (skips the autotuner)")
     ap.add_argument("--only", choices=("all", "batched", "cloth", "copy", "gas", "cloth32"), default="all",
                     help="run one part alone (for profiler runs); only 'all' prints the JSON line")
+    ap.add_argument("--cpu-leg", type=int, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg is not None:
+        # the CPU baselines' child process: pinned to one core before it loads
+        # anything, it never touches the GPU
+        os.sched_setaffinity(0, {args.cpu_leg})
+        print(json.dumps(cpu_leg()), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box of this run has one MI355X; --gpus {args.gpus} is not run. "
               "Scenes are independent, so N GPUs would each take 1/N of them (no exchange).", file=sys.stderr)
@@ -454,18 +536,19 @@ def main():
                                           seed=5, drain_limit=45.0)
         log(f"serving (sustained load): {serving_sustained}")
 
-    cpu = None if args.no_cpu else cpu_baseline()
-    if cpu:
-        cpu["cpu_model"] = cpu_model()
-        cpu["host_cpus"] = os.cpu_count()
-        log(f"cpu: {cpu}")
-    if cpu and (gas or cloth32):
-        plain_cpu = cpu_plain_runs()
+    cpus = None if args.no_cpu else cpu_baselines()
+    cpu = None
+    if cpus:
+        log(f"cpu: {json.dumps(cpus)}")
+        cpu = dict(cpus["softbody4096_all_pairs"])
+        cpu["candidate_list"] = cpus["softbody4096_candidate_list"]
+        if cloth:
+            cloth["cpu_baseline"] = cpus["cloth_1024"]
         if gas:
-            gas["cpu_baseline"] = plain_cpu["gas_262144"]
+            gas["cpu_baseline"] = dict(cpus["gas_262144_all_pairs"])
+            gas["cpu_baseline"]["candidate_list"] = cpus["gas_262144_candidate_list"]
         if cloth32:
-            cloth32["cpu_baseline"] = plain_cpu["cloth_32"]
-        log(f"cpu (plain path figures): {plain_cpu}")
+            cloth32["cpu_baseline"] = cpus["cloth_32"]
 
     # dominant kernel: one launch (graph replay) of k_scenes, both size classes
     # running concurrently on their streams, timed by HIP events on the batch's

@@ -6,7 +6,7 @@ builders and the build script.
 """
 from ._lib import (BD_ENGINE_AUTO, BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BD_ORDER_COLOUR,
                    BD_ORDER_PARTICLE, BendyError, lib)
-from .solver import Batch, Server, Solver, plan_layers, polygon_circle, polygon_new
+from .solver import Batch, SceneImage, Server, Solver, plan_layers, polygon_circle, polygon_new
 
 __all__ = ["Batch", "Server", "Solver", "plan_layers", "polygon_circle", "polygon_new", "lib", "BendyError",
            "BD_ENGINE_AUTO", "BD_ENGINE_GLOBAL", "BD_ENGINE_SCENE", "BD_ENGINE_TILED",

@@ -94,6 +94,11 @@ BD_ORDER_COLOUR = 0
 BD_ORDER_PARTICLE = 1
 
 
+class SceneImageInfo(Structure):
+    _fields_ = [("image_bytes", c_uint64), ("workspace_bytes", c_uint64), ("n_bodies", c_uint32),
+                ("n_circles", c_uint32), ("workgroup", c_uint32), ("lds_bytes", c_uint32)]
+
+
 class Stats(Structure):
     _fields_ = [("n_points", c_uint64), ("n_links", c_uint64), ("link_layers", c_uint64),
                 ("updates", c_uint64), ("substeps", c_uint64), ("kernel_launches", c_uint64),
@@ -112,6 +117,8 @@ _SIGS = [
     ("bd_solver_new", _S, []),
     ("bd_solver_free", None, [_S]),
     ("bd_solver_clone", _S, [_S]),
+    ("bd_solver_save", c_int64, [_S, c_void_p, c_uint64]),
+    ("bd_solver_load", _S, [c_void_p, c_uint64]),
     ("bd_set_gravity", None, [_S, Vec2]),
     ("bd_get_gravity", Vec2, [_S]),
     ("bd_set_bounds", None, [_S, Bounds]),
@@ -195,6 +202,11 @@ _SIGS = [
     ("bd_server_read_scene", c_int32, [_S, c_int64, _S]),
     ("bd_server_get_stats", c_int32, [_S, POINTER(ServerStats)]),
     ("bd_plan_layers", c_int64, [c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p]),
+    ("bd_scene_image_describe", c_int32, [_S, POINTER(SceneImageInfo)]),
+    ("bd_scene_image_write", c_int32, [_S, c_void_p, c_uint64]),
+    ("bd_scene_image_update", c_int32, [POINTER(SceneImageInfo), c_void_p, c_void_p, c_uint64, c_float, c_uint32,
+                                        c_void_p]),
+    ("bd_scene_image_read", c_int32, [_S, c_void_p, c_uint64]),
 ]
 
 _lib = None

@@ -28,6 +28,7 @@ SOURCES = [
     "engine_global.hip",
     "engine_scene.hip",
     "engine_tiled.hip",
+    "scene_image.hip",
 ]
 
 COMMON = [

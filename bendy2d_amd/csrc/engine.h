@@ -49,6 +49,14 @@ Engine *make_global_engine(hipStream_t stream);
 Engine *make_scene_engine(hipStream_t stream);
 Engine *make_tiled_engine(hipStream_t stream, uint32_t tile_points, uint32_t wg);
 
+// A download writes the device's bodies into the host scene by index; the
+// host scene must still have the shape the engine was uploaded with.
+inline bool host_shape_matches(const HostScene &s, uint64_t Np, uint64_t Nc, uint64_t Ng, std::string &err) {
+    if (s.p_pos.size() == Np && s.c_pos.size() == Nc && s.g_pos.size() == Ng) return true;
+    err = "host scene changed shape since the device copy was made";
+    return false;
+}
+
 // True when the SCENE engine (one workgroup, LDS-resident) can run this plan.
 bool scene_engine_fits(const HostScene &s, const Plan &p);
 bool tiled_engine_fits(const HostScene &s, const Plan &p);

@@ -1203,6 +1203,7 @@ public:
     }
 
     int download(HostScene &s, std::string &err) override {
+        if (!host_shape_matches(s, Np_, Nc_, Ng_, err)) return BD_ERR_STATE;
         if (N_) {
             std::vector<float2> pos(N_), prev(N_);
             BD_HIP(hipMemcpyAsync(pos.data(), pos_.p, N_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));

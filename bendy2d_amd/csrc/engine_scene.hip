@@ -870,7 +870,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     __syncthreads();
 
     // circle-pair workspace
-    char *ws = base + d.ws_off;
+    char *ws = d.ws_base ? (char *)d.ws_base : base + d.ws_off;
     const CCLayout cl = cc_layout(Nc, d.T);
     CCWork w;
     w.ref = (float2 *)(ws + cl.ref); w.save = (float2 *)(ws + cl.save);
@@ -1281,6 +1281,7 @@ public:
     }
 
     int download(HostScene &s, std::string &err) override {
+        if (!host_shape_matches(s, Np_, N_ - Np_, 0, err)) return BD_ERR_STATE;
         std::vector<float2> pos(N_), prev(N_);
         SceneState stt;
         if (N_) {

@@ -233,6 +233,7 @@ public:
     }
 
     int download(HostScene &s, std::string &err) override {
+        if (!host_shape_matches(s, N_, 0, 0, err)) return BD_ERR_STATE;
         if (!N_) return BD_OK;
         std::vector<float2> pos(N_), prev(N_);
         BD_HIP(hipMemcpyAsync(pos.data(), pos_[cur_].p, N_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));

@@ -46,7 +46,11 @@ struct SceneDesc {
                                  // end of an update (0: run all `steps`); SceneState::last_steps
                                  // tells how many ran
     float runi;                  // every circle's radius, when kFlagUniformR is set
+    uint32_t pad0;
+    uint64_t ws_base;            // device address of the circle-pair workspace when it is not
+                                 // in the chunk (0: base + ws_off)
 };
+static_assert(sizeof(SceneDesc) <= 256, "a scene descriptor fits its 256-byte slot");
 
 constexpr uint32_t kFlagPins = 1u;        // some particle is pinned
 constexpr uint32_t kFlagAcc = 2u;         // initial circle acc pending (first substep)

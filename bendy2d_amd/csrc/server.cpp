@@ -94,6 +94,7 @@ struct ServedScene {
     std::deque<Request> queue;
     uint64_t last_used = 0;
     uint64_t updates_run = 0;
+    bool failed = false;            // its circle-pair schedule failed: requests finish with BD_ERR_COLLISION
 };
 
 }  // namespace
@@ -176,6 +177,11 @@ static bool admit(bd_server *S, ServedScene &sc, const std::vector<ServedScene *
     sc.slot = S->free_slots.back();
     S->free_slots.pop_back();
     sc.off = off;
+    // the circle-pair workspace is not kept across eviction (only the chunk's
+    // initialised part and the state travel), so its lists and per-circle
+    // skin history start over; results do not depend on them
+    sc.saved.list_valid = 0u;
+    sc.saved.ws_ready = 0u;
     if (hipMemcpyAsync(S->arena.base + off, sc.host.data(), sc.init_bytes, hipMemcpyHostToDevice, S->core.stream) != hipSuccess ||
         hipMemcpyAsync(S->slots.p + sc.slot, &sc.saved, sizeof(SceneState), hipMemcpyHostToDevice, S->core.stream) != hipSuccess ||
         hipStreamSynchronize(S->core.stream) != hipSuccess) {
@@ -208,6 +214,36 @@ static void scheduler(bd_server *S) {
                 if (!kv.second->queue.empty()) run.push_back(kv.second.get());
         }
         if (run.empty()) continue;
+        // a scene whose schedule failed runs no more: its requests finish with
+        // the error (the kernel would skip its circle pairs, so results would
+        // no longer be the reference's)
+        {
+            std::vector<bd_request_done> fin;
+            const double t_now = S->now();
+            for (ServedScene *sc : run) {
+                if (!sc->failed) continue;
+                while (!sc->queue.empty()) {
+                    const Request &rq = sc->queue.front();
+                    bd_request_done d{};
+                    d.request_id = rq.id; d.scene_id = sc->id; d.steps = rq.steps; d.status = BD_ERR_COLLISION;
+                    d.t_submit = rq.t_submit; d.t_start = rq.t_start < 0 ? t_now : rq.t_start; d.t_done = t_now;
+                    fin.push_back(d);
+                    sc->queue.pop_front();
+                }
+            }
+            if (!fin.empty()) {
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    for (auto &d : fin) S->done.push_back(d);
+                    S->n_done += fin.size();
+                    S->pending -= fin.size();
+                }
+                S->idle_cv.notify_all();
+                run.erase(std::remove_if(run.begin(), run.end(), [](const ServedScene *sc) { return sc->failed; }),
+                          run.end());
+                if (run.empty()) continue;
+            }
+        }
         std::sort(run.begin(), run.end(), [](const ServedScene *a, const ServedScene *b) {
             return a->queue.front().t_submit < b->queue.front().t_submit ||
                    (a->queue.front().t_submit == b->queue.front().t_submit && a->id < b->id);
@@ -248,38 +284,42 @@ static void scheduler(bd_server *S) {
         int rc = c.run(1, spl, err);
         if (rc == BD_OK) rc = c.finish_timing(err);
         if (rc != BD_OK) { S->fatal = err; return; }
-        // with a time budget, each scene reports the updates it ran
-        uint32_t lo_slot = 0;
-        if (S->budget_ticks) {
-            uint32_t hi_slot = 0;
-            lo_slot = UINT32_MAX;
-            for (ServedScene *sc : launch) { lo_slot = std::min(lo_slot, sc->slot); hi_slot = std::max(hi_slot, sc->slot); }
-            S->readback.resize(hi_slot - lo_slot + 1);
-            if (hipMemcpyAsync(S->readback.data(), S->slots.p + lo_slot, S->readback.size() * sizeof(SceneState),
-                               hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-                hipStreamSynchronize(c.stream) != hipSuccess) {
-                S->fatal = "scene state readback failed";
-                return;
-            }
+        // every launched scene reports whether its schedule failed and, with a
+        // time budget, the updates it ran
+        uint32_t lo_slot = UINT32_MAX, hi_slot = 0;
+        for (ServedScene *sc : launch) { lo_slot = std::min(lo_slot, sc->slot); hi_slot = std::max(hi_slot, sc->slot); }
+        S->readback.resize(hi_slot - lo_slot + 1);
+        if (hipMemcpyAsync(S->readback.data(), S->slots.p + lo_slot, S->readback.size() * sizeof(SceneState),
+                           hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess) {
+            S->fatal = "scene state readback failed";
+            return;
         }
         const double t_done = S->now();
         std::vector<bd_request_done> fin;
         uint64_t pu = 0;
         for (ServedScene *sc : launch) {
             Request &rq = sc->queue.front();
+            const SceneState &ss = S->readback[sc->slot - lo_slot];
             uint32_t ran = std::min(rq.remaining, spl);
-            if (S->budget_ticks && sc->desc.N)
-                ran = std::max(1u, std::min(ran, S->readback[sc->slot - lo_slot].last_steps));
+            if (S->budget_ticks && sc->desc.N) ran = std::max(1u, std::min(ran, ss.last_steps));
             rq.remaining -= ran;
             sc->updates_run += ran;
             sc->acc_pending = false;
             sc->last_used = S->launches;
             pu += (uint64_t)sc->desc.N * sc->params.sub_steps * ran;
+            if (ss.err) {
+                // the request ends here with the error; the scene's later
+                // requests are failed before the next launch
+                sc->failed = true;
+                rq.remaining = 0;
+            }
             if (rq.remaining == 0) {
                 bd_request_done d{};
-                d.request_id = rq.id; d.scene_id = sc->id; d.steps = rq.steps; d.status = BD_OK;
+                d.request_id = rq.id; d.scene_id = sc->id; d.steps = rq.steps;
+                d.status = sc->failed ? BD_ERR_COLLISION : BD_OK;
                 d.t_submit = rq.t_submit; d.t_start = rq.t_start; d.t_done = t_done;
-                d.particle_updates = (uint64_t)sc->desc.N * sc->params.sub_steps * rq.steps;
+                d.particle_updates = sc->failed ? 0 : (uint64_t)sc->desc.N * sc->params.sub_steps * rq.steps;
                 fin.push_back(d);
                 sc->queue.pop_front();
             }

@@ -94,10 +94,21 @@ static int ensure_host(const bd_solver *cs) {
 }
 
 // Structural change: host must be current and the plan is rebuilt next update.
-static void begin_change(bd_solver *s) {
-    ensure_host(s);
+// When the device copy cannot be brought back, nothing changes and the error
+// is returned (the host copy would be stale, and the device copy would no
+// longer match the host scene's shape).
+static int begin_change(bd_solver *s) {
+    int rc = ensure_host(s);
+    if (rc) return rc;
     s->plan_valid = false;
+    return BD_OK;
 }
+
+#define BD_BEGIN_CHANGE(s)                   \
+    do {                                     \
+        int rc_ = begin_change(s);           \
+        if (rc_) return rc_;                 \
+    } while (0)
 
 static uint32_t idx32(uint64_t v) { return v >= 0xffffffffull ? kBadIndex : (uint32_t)v; }
 
@@ -208,7 +219,7 @@ int32_t bd_get_bounds_active(const bd_solver *s) { return s ? s->host.bounds_act
 
 int64_t bd_add_particle(bd_solver *s, bd_vec2 pos) {
     if (!s) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.p_pos.push_back(V2{pos.x, pos.y});
     s->host.p_prev.push_back(V2{pos.x, pos.y});
     s->host.p_pin.push_back(0);
@@ -217,7 +228,7 @@ int64_t bd_add_particle(bd_solver *s, bd_vec2 pos) {
 
 int64_t bd_ext_add_particle_with_prev(bd_solver *s, bd_vec2 pos, bd_vec2 prev) {
     if (!s) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.p_pos.push_back(V2{pos.x, pos.y});
     s->host.p_prev.push_back(V2{prev.x, prev.y});
     s->host.p_pin.push_back(0);
@@ -226,7 +237,7 @@ int64_t bd_ext_add_particle_with_prev(bd_solver *s, bd_vec2 pos, bd_vec2 prev) {
 
 int64_t bd_ext_add_particles(bd_solver *s, const bd_vec2 *pos, const bd_vec2 *prev, uint64_t n) {
     if (!s || (!pos && n)) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     int64_t first = (int64_t)s->host.p_pos.size();
     s->host.p_pos.reserve(first + n); s->host.p_prev.reserve(first + n);
     for (uint64_t i = 0; i < n; i++) {
@@ -240,7 +251,7 @@ int64_t bd_ext_add_particles(bd_solver *s, const bd_vec2 *pos, const bd_vec2 *pr
 
 int64_t bd_add_circle(bd_solver *s, const bd_circle *c) {
     if (!s || !c) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.c_pos.push_back(V2{c->point.pos.x, c->point.pos.y});
     s->host.c_prev.push_back(V2{c->point.prev_pos.x, c->point.prev_pos.y});
     s->host.c_acc.push_back(V2{c->point.acc.x, c->point.acc.y});
@@ -250,7 +261,7 @@ int64_t bd_add_circle(bd_solver *s, const bd_circle *c) {
 
 int64_t bd_ext_add_circles(bd_solver *s, const bd_circle *c, uint64_t n) {
     if (!s || (!c && n)) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     int64_t first = (int64_t)s->host.c_pos.size();
     for (uint64_t i = 0; i < n; i++) {
         s->host.c_pos.push_back(V2{c[i].point.pos.x, c[i].point.pos.y});
@@ -263,14 +274,14 @@ int64_t bd_ext_add_circles(bd_solver *s, const bd_circle *c, uint64_t n) {
 
 int64_t bd_add_particle_link(bd_solver *s, const bd_link *l) {
     if (!s || !l) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.plinks.push_back(LinkH{idx32(l->particle_a), idx32(l->particle_b), l->target_distance});
     return (int64_t)s->host.plinks.size() - 1;
 }
 
 int64_t bd_add_circle_link(bd_solver *s, const bd_link *l) {
     if (!s || !l) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.clinks.push_back(LinkH{idx32(l->particle_a), idx32(l->particle_b), l->target_distance});
     return (int64_t)s->host.clinks.size() - 1;
 }
@@ -278,7 +289,7 @@ int64_t bd_add_circle_link(bd_solver *s, const bd_link *l) {
 static int64_t add_links_bulk(bd_solver *s, std::vector<LinkH> &dst, const uint32_t *a, const uint32_t *b,
                               const float *t, uint64_t n) {
     if (!s || ((!a || !b || !t) && n)) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     int64_t first = (int64_t)dst.size();
     dst.reserve(first + n);
     for (uint64_t i = 0; i < n; i++) dst.push_back(LinkH{a[i], b[i], t[i]});
@@ -296,7 +307,7 @@ int64_t bd_ext_add_circle_links(bd_solver *s, const uint32_t *a, const uint32_t 
 int64_t bd_add_polygon(bd_solver *s, const bd_polygon_in *p) {
     if (!s || !p || (!p->particles && p->n_particles) || (!p->particle_links && p->n_links)) return BD_ERR_ARG;
     if (p->n_particles >= 0xffffffffull || p->n_links >= 0xffffffffull) return fail(BD_ERR_CAPACITY, "polygon too large");
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     HostScene &h = s->host;
     Poly pg;
     pg.off = (uint32_t)h.g_pos.size();
@@ -457,7 +468,7 @@ uint32_t bd_ext_get_link_iterations(const bd_solver *s) { return s ? s->host.lin
 int32_t bd_ext_set_pinned(bd_solver *s, uint64_t i, int32_t pinned) {
     if (!s) return BD_ERR_ARG;
     if (i >= s->host.p_pos.size()) return BD_ERR_INDEX;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.p_pin[i] = pinned ? 1 : 0;
     return BD_OK;
 }
@@ -465,21 +476,21 @@ int32_t bd_ext_set_pinned(bd_solver *s, uint64_t i, int32_t pinned) {
 int32_t bd_ext_set_pinned_range(bd_solver *s, uint64_t first, const uint8_t *pinned, uint64_t n) {
     if (!s || (!pinned && n)) return BD_ERR_ARG;
     if (first + n > s->host.p_pos.size()) return BD_ERR_INDEX;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     for (uint64_t i = 0; i < n; i++) s->host.p_pin[first + i] = pinned[i] ? 1 : 0;
     return BD_OK;
 }
 
 int64_t bd_ext_add_static_circle(bd_solver *s, bd_vec2 c, float r) {
     if (!s) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->host.cols.push_back(Collider{0, c.x, c.y, r, 0, 0});
     return (int64_t)s->host.cols.size() - 1;
 }
 
 int64_t bd_ext_add_static_polygon(bd_solver *s, const bd_vec2 *v, uint64_t n) {
     if (!s || !v || n < 3) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     std::vector<V2> pts(n);
     for (uint64_t i = 0; i < n; i++) pts[i] = V2{v[i].x, v[i].y};
     uint32_t off = (uint32_t)(s->host.planes.size() / 3);
@@ -514,7 +525,7 @@ int32_t bd_ext_get_pinned_range(const bd_solver *s, uint64_t first, uint8_t *out
 
 int32_t bd_ext_set_engine(bd_solver *s, int32_t engine) {
     if (!s || engine < BD_ENGINE_AUTO || engine > BD_ENGINE_TILED) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->engine_pref = engine;
     return BD_OK;
 }
@@ -524,7 +535,7 @@ int32_t bd_ext_get_engine_used(const bd_solver *s) { return s ? s->engine_used :
 int32_t bd_ext_set_tiling(bd_solver *s, uint32_t tile_points, uint32_t workgroup) {
     if (!s || tile_points < 64 || tile_points > 16384) return BD_ERR_ARG;
     if (workgroup != 256 && workgroup != 512 && workgroup != 1024) return BD_ERR_ARG;
-    begin_change(s);
+    BD_BEGIN_CHANGE(s);
     s->tile_points = tile_points;
     s->tile_wg = workgroup;
     return BD_OK;
@@ -563,6 +574,8 @@ static int ensure_device(bd_solver *s) {
     return BD_OK;
 }
 
+static void settle_acc(bd_solver *s, uint64_t substeps);
+
 int32_t bd_update(bd_solver *s, float dt) {
     if (!s) return BD_ERR_ARG;
     std::string err;
@@ -600,11 +613,25 @@ int32_t bd_update(bd_solver *s, float dt) {
         s->engine_used = want;
     }
     StepParams sp = make_params(s->host, dt, s->acc_pending);
+    const uint64_t substeps_before = s->stats.substeps;
     int rc = s->eng->update(sp, s->stats, err);
-    if (rc) return fail(rc, err);
+    if (rc) {
+        // The substeps that completed stay applied on the device (getters show
+        // them); the initial circle / polygon-point acc is spent once one ran.
+        if (s->stats.substeps > substeps_before) settle_acc(s, s->stats.substeps - substeps_before);
+        s->dev_newer = true;
+        return fail(rc, err);
+    }
     s->stats.updates++;
-    // acc after update_positions: zero, except static polygons, which skip
-    // Polygon::update and keep adding gravity (src/polygon.rs:125-128).
+    settle_acc(s, s->host.sub_steps);
+    s->dev_newer = true;
+    return BD_OK;
+}
+
+// acc after update_positions: zero, except static polygons, which skip
+// Polygon::update and keep adding gravity (src/polygon.rs:125-128); a static
+// polygon's points gain one gravity per substep that ran.
+static void settle_acc(bd_solver *s, uint64_t substeps) {
     HostScene &h = s->host;
     for (V2 &a : h.c_acc) a = V2{0.0f, 0.0f};
     for (size_t p = 0; p < h.polys.size(); p++) {
@@ -612,7 +639,7 @@ int32_t bd_update(bd_solver *s, float dt) {
         for (uint32_t i = 0; i < pg.n; i++) {
             V2 &a = h.g_acc[pg.off + i];
             if (pg.is_static) {
-                for (uint32_t k = 0; k < h.sub_steps; k++) a = V2{a.x + h.gravity.x, a.y + h.gravity.y};
+                for (uint64_t k = 0; k < substeps; k++) a = V2{a.x + h.gravity.x, a.y + h.gravity.y};
             } else {
                 a = V2{0.0f, 0.0f};
             }
@@ -620,9 +647,136 @@ int32_t bd_update(bd_solver *s, float dt) {
         if (p < s->center_from_prev.size()) s->center_from_prev[p] = pg.is_static ? 0 : 1;
     }
     s->acc_pending = false;
-    s->dev_newer = true;
-    return BD_OK;
 }
+
+}  // extern "C"
+
+// ---- snapshot format --------------------------------------------------------
+//
+// A versioned, self-contained byte image of a solver (little-endian):
+//   "BD2DSNAP" | u32 version | u32 header words | header (u32/f32 words) |
+//   sections: particles (pos, prev, pin), particle links, circles (pos, prev,
+//   acc, radius), circle links, polygons (records, points, links), static
+//   colliders (records, planes, vertices), per-polygon centre flags |
+//   u64 FNV-1a of every preceding byte.
+// Only the host scene is saved: the device copy is brought back first, and a
+// loaded solver plans and uploads on its next update, so a run continued from
+// a snapshot equals the uninterrupted run bit for bit.
+
+namespace {
+
+constexpr char kSnapMagic[8] = {'B', 'D', '2', 'D', 'S', 'N', 'A', 'P'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapWriter {
+    std::vector<uint8_t> b;
+    template <typename T> void put(const T &v) {
+        const uint8_t *p = (const uint8_t *)&v;
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    template <typename T> void arr(const std::vector<T> &v) {
+        const uint8_t *p = (const uint8_t *)v.data();
+        b.insert(b.end(), p, p + sizeof(T) * v.size());
+    }
+};
+
+struct SnapReader {
+    const uint8_t *p, *end;
+    bool ok = true;
+    template <typename T> T get() {
+        T v{};
+        if ((size_t)(end - p) < sizeof(T)) { ok = false; return v; }
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        return v;
+    }
+    template <typename T> void arr(std::vector<T> &v, uint64_t n) {
+        if (!ok || n > (uint64_t)(end - p) / sizeof(T)) { ok = false; return; }
+        v.resize(n);
+        std::memcpy(v.data(), p, sizeof(T) * n);
+        p += sizeof(T) * n;
+    }
+};
+
+uint64_t fnv1a(const uint8_t *p, size_t n) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < n; i++) { h ^= p[i]; h *= 0x100000001b3ull; }
+    return h;
+}
+
+}  // namespace
+
+extern "C" int64_t bd_solver_save(const bd_solver *s, void *buf, uint64_t cap) {
+    if (!s) return BD_ERR_ARG;
+    int rc = ensure_host(s);
+    if (rc) return rc;
+    const HostScene &h = s->host;
+    SnapWriter w;
+    w.b.insert(w.b.end(), kSnapMagic, kSnapMagic + 8);
+    w.put(kSnapVersion);
+    const uint32_t header_words = 21;
+    w.put(header_words);
+    w.put(h.gravity); w.put(h.bpos); w.put(h.bsize);                        // 6 words
+    w.put(h.bounds_active); w.put(h.sub_steps); w.put(h.link_iters);        // 3
+    w.put(s->engine_pref); w.put((uint32_t)s->acc_pending);                 // 2
+    w.put((uint32_t)h.p_pos.size()); w.put((uint32_t)h.plinks.size());      // 2
+    w.put((uint32_t)h.c_pos.size()); w.put((uint32_t)h.clinks.size());      // 2
+    w.put((uint32_t)h.polys.size()); w.put((uint32_t)h.g_pos.size());      // 2
+    w.put((uint32_t)h.glinks.size()); w.put((uint32_t)h.cols.size());      // 2
+    w.put((uint32_t)h.planes.size()); w.put((uint32_t)h.verts.size());     // 2
+    w.arr(h.p_pos); w.arr(h.p_prev); w.arr(h.p_pin); w.arr(h.plinks);
+    w.arr(h.c_pos); w.arr(h.c_prev); w.arr(h.c_acc); w.arr(h.c_r); w.arr(h.clinks);
+    w.arr(h.polys); w.arr(h.g_pos); w.arr(h.g_prev); w.arr(h.g_acc); w.arr(h.glinks);
+    w.arr(h.cols); w.arr(h.planes); w.arr(h.verts);
+    std::vector<uint8_t> cfp(h.polys.size(), 0);
+    for (size_t p = 0; p < cfp.size() && p < s->center_from_prev.size(); p++) cfp[p] = s->center_from_prev[p];
+    w.arr(cfp);
+    w.put(fnv1a(w.b.data(), w.b.size()));
+    if (buf && cap >= w.b.size()) std::memcpy(buf, w.b.data(), w.b.size());
+    else if (buf) return fail(BD_ERR_ARG, "buffer smaller than the snapshot");
+    return (int64_t)w.b.size();
+}
+
+extern "C" bd_solver *bd_solver_load(const void *buf, uint64_t bytes) {
+    if (!buf || bytes < 8 + 8 + 8) { set_last_error("not a solver snapshot"); return nullptr; }
+    const uint8_t *b = (const uint8_t *)buf;
+    if (std::memcmp(b, kSnapMagic, 8) != 0) { set_last_error("not a solver snapshot"); return nullptr; }
+    uint64_t sum;
+    std::memcpy(&sum, b + bytes - 8, 8);
+    if (fnv1a(b, bytes - 8) != sum) { set_last_error("snapshot checksum mismatch"); return nullptr; }
+    SnapReader r{b + 8, b + bytes - 8};
+    const uint32_t version = r.get<uint32_t>(), words = r.get<uint32_t>();
+    if (version != kSnapVersion || words != 21) { set_last_error("unsupported snapshot version"); return nullptr; }
+    std::unique_ptr<bd_solver> s(bd_solver_new());
+    if (!s) return nullptr;
+    HostScene &h = s->host;
+    h.gravity = r.get<V2>(); h.bpos = r.get<V2>(); h.bsize = r.get<V2>();
+    h.bounds_active = r.get<int32_t>(); h.sub_steps = r.get<uint32_t>(); h.link_iters = r.get<uint32_t>();
+    s->engine_pref = r.get<int32_t>();
+    s->acc_pending = r.get<uint32_t>() != 0;
+    const uint32_t np = r.get<uint32_t>(), npl = r.get<uint32_t>(), nc = r.get<uint32_t>(), ncl = r.get<uint32_t>();
+    const uint32_t npoly = r.get<uint32_t>(), ng = r.get<uint32_t>(), ngl = r.get<uint32_t>(), ncol = r.get<uint32_t>();
+    const uint32_t nplanes = r.get<uint32_t>(), nverts = r.get<uint32_t>();
+    r.arr(h.p_pos, np); r.arr(h.p_prev, np); r.arr(h.p_pin, np); r.arr(h.plinks, npl);
+    r.arr(h.c_pos, nc); r.arr(h.c_prev, nc); r.arr(h.c_acc, nc); r.arr(h.c_r, nc); r.arr(h.clinks, ncl);
+    r.arr(h.polys, npoly); r.arr(h.g_pos, ng); r.arr(h.g_prev, ng); r.arr(h.g_acc, ng); r.arr(h.glinks, ngl);
+    r.arr(h.cols, ncol); r.arr(h.planes, nplanes); r.arr(h.verts, nverts);
+    r.arr(s->center_from_prev, npoly);
+    bool ok = r.ok && r.p == r.end && h.sub_steps >= 1 && h.sub_steps <= 65535 && h.link_iters >= 1 &&
+              s->engine_pref >= BD_ENGINE_AUTO && s->engine_pref <= BD_ENGINE_TILED && nplanes % 3 == 0;
+    for (const Poly &pg : h.polys)
+        ok = ok && (uint64_t)pg.off + pg.n <= ng && (uint64_t)pg.loff + pg.nl <= ngl;
+    for (const Collider &c : h.cols)
+        ok = ok && (c.kind == 0 || (c.kind == 1 && (uint64_t)c.plane_off + c.nv <= nplanes / 3 &&
+                                    2ull * ((uint64_t)c.plane_off + c.nv) <= nverts));
+    if (!ok) { set_last_error("malformed snapshot"); return nullptr; }
+    // the first update after loading plans and uploads; the initial acc of
+    // circles and polygon points is pending only if no update has run
+    s->acc_pending = false;
+    return s.release();
+}
+
+extern "C" {
 
 int32_t bd_ext_sync(bd_solver *s) {
     if (!s) return BD_ERR_ARG;

@@ -76,6 +76,22 @@ class Solver:
             raise RuntimeError("bd_solver_clone failed")
         return Solver(h)
 
+    def save(self):
+        """The solver's snapshot (bd_solver_save) as bytes."""
+        n = check(lib().bd_solver_save(self._h, None, 0), "bd_solver_save")
+        buf = (ctypes.c_uint8 * n)()
+        check(lib().bd_solver_save(self._h, buf, n), "bd_solver_save")
+        return bytes(buf)
+
+    @staticmethod
+    def load(data):
+        """A new solver from a snapshot made by save()."""
+        buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+        h = lib().bd_solver_load(buf, len(data))
+        if not h:
+            raise _lib.BendyError(_lib.BD_ERR_ARG, "bd_solver_load")
+        return Solver(h)
+
     # --- public fields (src/solver.rs:21-23) ---
     @property
     def gravity(self):
@@ -435,3 +451,29 @@ class Server:
         s = ServerStats()
         check(lib().bd_server_get_stats(self._h, ctypes.byref(s)), "server_get_stats")
         return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+
+class SceneImage:
+    """One scene in caller-owned device memory (bd_scene_image_*): `info`
+    describes its sizes and launch shape, `host` holds the image bytes to copy
+    to a 256-byte-aligned device buffer.  update() queues the kernels on a
+    caller stream and does not synchronise or allocate."""
+
+    def __init__(self, solver):
+        self.info = _lib.SceneImageInfo()
+        check(lib().bd_scene_image_describe(solver._h, ctypes.byref(self.info)), "bd_scene_image_describe")
+        self.host = np.zeros(self.info.image_bytes, np.uint8)
+        check(lib().bd_scene_image_write(solver._h, _ptr(self.host), self.host.nbytes), "bd_scene_image_write")
+
+    @property
+    def workspace_bytes(self):
+        return self.info.workspace_bytes
+
+    def update(self, image_ptr, workspace_ptr, workspace_bytes, dt, steps, stream=None):
+        check(lib().bd_scene_image_update(ctypes.byref(self.info), image_ptr, workspace_ptr, int(workspace_bytes),
+                                          float(dt), int(steps), stream), "bd_scene_image_update")
+
+    @staticmethod
+    def read(solver, host_bytes):
+        """Positions from a host copy of an image into `solver` (holding the same scene)."""
+        check(lib().bd_scene_image_read(solver._h, _ptr(host_bytes), host_bytes.nbytes), "bd_scene_image_read")

@@ -103,6 +103,16 @@ void bd_solver_free(bd_solver *s);
 /* #[derive(Clone)] on Solver (src/solver.rs:19): deep copy of the state. */
 bd_solver *bd_solver_clone(const bd_solver *s);
 
+/* Snapshot: the whole solver (bodies, links, colliders, public fields and
+ * extension settings) as a versioned byte image with a checksum; the
+ * reference's only analogue is #[derive(Clone)] on Solver (src/solver.rs:19).
+ * bd_solver_save returns the snapshot's size in bytes (buf NULL: size only)
+ * or a negative BD_ERR_*; bd_solver_load returns a new solver, or NULL
+ * (bd_last_error) when the bytes are not a valid snapshot.  A run continued
+ * from a loaded snapshot equals the uninterrupted run bit for bit. */
+int64_t bd_solver_save(const bd_solver *s, void *buf, uint64_t cap);
+bd_solver *bd_solver_load(const void *buf, uint64_t bytes);
+
 /* pub gravity / pub bounds / pub bounds_active (src/solver.rs:21-23).
  * bounds_active is stored and returned but, as in the reference, never read
  * by bd_update (src/solver.rs:155-165). */
@@ -157,6 +167,10 @@ uint64_t bd_get_polygon_links(const bd_solver *s, uint64_t index, bd_link *out, 
 /* Solver::update(dt): src/solver.rs:106-116.  Runs sub_steps substeps of
  * gravity -> links -> dynamic collisions -> bounds -> integrate on the GPU.
  * Returns BD_OK or a negative BD_ERR_*; on BD_ERR_LINK nothing has moved.
+ * On a failure during the step (BD_ERR_COLLISION, BD_ERR_DEVICE) the
+ * substeps that completed stay applied and getters show them.  Adders and
+ * other structural changes return the error instead of changing anything
+ * while the device state cannot be read back.
  * The call is asynchronous with respect to the host except where the circle
  * pair schedule needs a readback; getters synchronise. */
 int32_t bd_update(bd_solver *s, float dt);
@@ -347,7 +361,10 @@ int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t 
  * order.  When the budget is full, idle scenes (no pending request) are
  * evicted to host memory, least recently used first, and brought back when a
  * request names them; a new scene that cannot fit waits.  Results are
- * bit-identical to bd_update on the same scene, alone or in any mix. */
+ * bit-identical to bd_update on the same scene, alone or in any mix.  A scene
+ * whose circle-pair schedule fails (an engine capacity limit) stops running:
+ * the request in flight and every later request for it finish with status
+ * BD_ERR_COLLISION and particle_updates 0. */
 typedef struct bd_server bd_server;
 
 typedef struct bd_server_config {
@@ -392,6 +409,45 @@ int32_t bd_server_wait_idle(bd_server *srv, double timeout_s);
 /* Copy a scene's current state into `out` (a solver holding the same scene). */
 int32_t bd_server_read_scene(bd_server *srv, int64_t scene_id, bd_solver *out);
 int32_t bd_server_get_stats(bd_server *srv, bd_server_stats *out);
+
+/* ------------------------------------------------------------------ */
+/* Scene images: one scene stepped in caller-owned device memory        */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference.  For callers that manage device memory
+ * and streams themselves (code under HIP graph capture, serving loops): a
+ * scene image is one scene (no soft polygons, 1 to 16384 bodies) in the
+ * SCENE engine's device layout behind a 512-byte header.  The caller copies
+ * the host image to device memory it owns (256-byte aligned) and advances it
+ * with bd_scene_image_update on its own stream: two kernel launches, no
+ * allocation, no host copy, no synchronisation, so the call can be captured
+ * in a HIP graph and replayed.  The circle-pair workspace is a second caller
+ * buffer (16-byte aligned) of at least workspace_bytes; its contents do not
+ * carry over between calls, so one workspace can serve any number of images
+ * as long as calls that share it do not overlap.  Results are bit-identical
+ * to bd_update on the same scene.  Copy the image back and pass it to
+ * bd_scene_image_read to see positions through a solver holding the scene. */
+typedef struct bd_scene_image_info {
+    uint64_t image_bytes;      /* device bytes of the image */
+    uint64_t workspace_bytes;  /* device bytes of workspace an update needs (0: none) */
+    uint32_t n_bodies, n_circles;
+    uint32_t workgroup;        /* launch shape for the scene: threads per workgroup */
+    uint32_t lds_bytes;        /* and dynamic LDS bytes */
+} bd_scene_image_info;
+
+int32_t bd_scene_image_describe(const bd_solver *s, bd_scene_image_info *info);
+/* Write the scene's image (its state now) into host memory of image_bytes. */
+int32_t bd_scene_image_write(const bd_solver *s, void *host_image, uint64_t bytes);
+/* Advance the image `steps` updates of dt on hip_stream (a hipStream_t; NULL:
+ * the null stream).  Returns once the kernels are queued.  A workspace
+ * smaller than the image needs is refused (BD_ERR_ARG), and, should the
+ * image header disagree with info, the device leaves the scene untouched and
+ * bd_scene_image_read reports BD_ERR_CAPACITY. */
+int32_t bd_scene_image_update(const bd_scene_image_info *info, void *image, void *workspace,
+                              uint64_t workspace_bytes, float dt, uint32_t steps, void *hip_stream);
+/* Copy positions from a host copy of an image into `s`, a solver holding the
+ * same scene (e.g. the one described).  BD_ERR_COLLISION if the scene's
+ * circle-pair schedule failed. */
+int32_t bd_scene_image_read(bd_solver *s, const void *host_image, uint64_t bytes);
 
 /* ------------------------------------------------------------------ */
 /* Scene generators (synthetic inputs of the benchmark workloads)       */

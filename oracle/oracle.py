@@ -85,6 +85,7 @@ def lib():
                 "law_particle_update": (None, [c_void_p, c_real]),
                 "law_line_intersection": (c_int, [c_void_p, c_void_p]),
                 "law_resolve_line": (c_int, [c_void_p, c_void_p]),
+                "all_pairs_rows": (c_int64, [c_void_p, c_int64, c_int64]),
             })
             for name, (res, args) in sig.items():
                 f = getattr(L, pfx + name)
@@ -142,6 +143,11 @@ class OracleSolver:
 
     def cc_redos(self):
         return self._f("cc_redos")(self._h)
+
+    def all_pairs_rows(self, first, last):
+        """Run rows [first, last) of the reference's all-pairs circle loop on
+        this solver's circles (moving them); returns the pair tests made."""
+        return self._f("all_pairs_rows")(self._h, int(first), int(last))
 
     def add_particle(self, pos, prev=None):
         if prev is None:

@@ -547,6 +547,20 @@ static void F(circles_all_pairs)(F(solver) *s) {
             F(circle_pair)(&s->circles[i], &s->circles[j], NULL, NULL);
 }
 
+/* Timing helper for the CPU baseline: the reference's all-pairs loop
+ * (src/solver.rs:168-177) for rows i in [first, last) only, on the solver's
+ * circles (which it moves).  Returns the pair tests made, so a caller can time
+ * a bounded slice of the loop and extrapolate to its nc (nc - 1) / 2 tests. */
+int64_t F(all_pairs_rows)(F(solver) *s, int64_t first, int64_t last) {
+    int64_t tests = 0;
+    for (int64_t i = first < 0 ? 0 : first; i < s->nc && i < last; i++)
+        for (int64_t j = i + 1; j < s->nc; j++) {
+            F(circle_pair)(&s->circles[i], &s->circles[j], NULL, NULL);
+            tests++;
+        }
+    return tests;
+}
+
 /* The same sweep restricted to candidate pairs (distance at sweep start below
  * r_i + r_j + margin), processed in the same lexicographic order.  A pair left
  * out can only matter if a circle moves by margin/2 or more during the sweep;
@@ -576,32 +590,41 @@ static void F(circles_candidates)(F(solver) *s) {
     if (nc < 2) return;
     T(v2) *save = (T(v2) *)malloc(sizeof(T(v2)) * (size_t)nc);
     REAL *path = (REAL *)malloc(sizeof(REAL) * (size_t)nc);
+    uint8_t *inert = (uint8_t *)malloc((size_t)nc);
     REAL rmax = 0;
-    int finite = 1;
+    int huge = 0;
     for (int64_t i = 0; i < nc; i++) {
         save[i] = s->circles[i].pt.pos;
         if (s->circles[i].r > rmax) rmax = s->circles[i].r;
-        if (!(fabs((double)save[i].x) < 1e15) || !(fabs((double)save[i].y) < 1e15)) finite = 0;
+        /* A circle with a NaN or infinite coordinate takes part in no pair:
+         * every difference with it is NaN or infinite, so q < rs*rs is false
+         * (src/circle.rs:145).  It stays out of the grid. */
+        inert[i] = !isfinite((double)save[i].x) || !isfinite((double)save[i].y);
+        if (!inert[i] && (!(fabs((double)save[i].x) < 1e15) || !(fabs((double)save[i].y) < 1e15))) huge = 1;
     }
-    if (!finite) { F(circles_all_pairs)(s); free(save); free(path); return; }
+    if (huge) { F(circles_all_pairs)(s); free(save); free(path); free(inert); return; }
     REAL margin = s->cc_margin > 0 ? s->cc_margin : (REAL)0.25f;
     F(keyed) *keys = (F(keyed) *)malloc(sizeof(F(keyed)) * (size_t)nc);
     for (int attempt = 0;; attempt++) {
         double cell = 2.0 * (double)rmax + (double)margin;
         if (!(cell > 0)) cell = 1.0;
+        int64_t nk = 0;
         for (int64_t i = 0; i < nc; i++) {
+            if (inert[i]) continue;
             int64_t cx = (int64_t)floor((double)save[i].x / cell), cy = (int64_t)floor((double)save[i].y / cell);
-            keys[i].key = (cy << 32) + cx; keys[i].i = i;   /* |cx|, |cy| < 2^31 for |pos| < 1e15 / cell */
+            keys[nk].key = (cy << 32) + cx; keys[nk].i = i;   /* |cx|, |cy| < 2^31 for |pos| < 1e15 / cell */
+            nk++;
         }
-        qsort(keys, (size_t)nc, sizeof(F(keyed)), F(keyed_cmp));
+        qsort(keys, (size_t)nk, sizeof(F(keyed)), F(keyed_cmp));
         int64_t npairs = 0, cap = nc * 4 + 16;
         F(pair) *pairs = (F(pair) *)malloc(sizeof(F(pair)) * (size_t)cap);
         for (int64_t i = 0; i < nc; i++) {
+            if (inert[i]) continue;
             int64_t cx = (int64_t)floor((double)save[i].x / cell), cy = (int64_t)floor((double)save[i].y / cell);
             for (int64_t yy = cy - 1; yy <= cy + 1; yy++) {
                 for (int64_t xx = cx - 1; xx <= cx + 1; xx++) {
                     int64_t key = (yy << 32) + xx;
-                    for (int64_t k = F(lower_key)(keys, nc, key); k < nc && keys[k].key == key; k++) {
+                    for (int64_t k = F(lower_key)(keys, nk, key); k < nk && keys[k].key == key; k++) {
                         int64_t j = keys[k].i;
                         if (j <= i) continue;
                         double dx = (double)save[i].x - save[j].x, dy = (double)save[i].y - save[j].y;
@@ -638,7 +661,7 @@ static void F(circles_candidates)(F(solver) *s) {
         margin = (REAL)(2.5 * pmax + (double)margin);
     }
     s->cc_margin = margin;
-    free(keys); free(save); free(path);
+    free(keys); free(save); free(path); free(inert);
 }
 
 /* solve_dynamic_collisions: src/solver.rs:167-188, then EXT4 static colliders */

@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from helpers import ROOT, bits_equal, gpu_available, to_oracle
+from helpers import ROOT, bits_equal, gpu_available, state, to_oracle
 from bendy2d_amd import Solver, lib, polygon_circle, polygon_new, workloads
 from bendy2d_amd import _lib
 from oracle import OracleSolver
@@ -148,3 +148,73 @@ def test_cpp_facade_program_builds_and_loads():
     assert os.path.exists(exe)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode in (0, 2), r.stdout + r.stderr
+
+
+def test_scene_image_host_roundtrip():
+    # bd_scene_image_describe / write / read need no device: an image read
+    # back before any update leaves the scene as it was
+    from bendy2d_amd import SceneImage
+    s = Solver()
+    workloads.softbody(s, 1234, max_side=12)
+    before = s.get_circles().copy()
+    img = SceneImage(s)
+    n = s.stats()["n_points"]
+    assert img.info.n_bodies == n and img.info.n_circles == n
+    assert img.info.workgroup in (64, 128, 256, 512, 1024)
+    assert img.info.image_bytes == len(img.host) and img.workspace_bytes > 0
+    c = s.clone()
+    SceneImage.read(c, img.host)
+    assert bits_equal(c.get_circles(), before)
+    with pytest.raises(_lib.BendyError):
+        SceneImage.read(c, img.host[:100])
+    bad = img.host.copy()
+    bad[256 + 12:256 + 16] = 0  # format version word
+    with pytest.raises(_lib.BendyError):
+        SceneImage.read(c, bad)
+    # soft polygons are not held by scene images
+    p = Solver()
+    p.add_polygon_circle(3.0, (5.0, 5.0), 6)
+    with pytest.raises(_lib.BendyError) as e:
+        SceneImage(p)
+    assert e.value.code == _lib.BD_ERR_CAPACITY
+
+
+def _mixed_scene():
+    s = Solver()
+    workloads.add_grid(s, 6, 5, x0=20.0, y0=90.0, pin_top=True)
+    workloads.add_grid(s, 4, 4, x0=60.0, y0=70.0, as_circles=True, radius=0.45)
+    s.add_circle((10.0, 10.0), 0.7, prev_pos=(9.9, 10.0), acc=(1.0, -2.0))
+    s.add_polygon_circle(6.0, (50.0, 40.0), 8, False)
+    s.add_polygon_new([(10.0, 80.0), (60.0, 80.0), (60.0, 90.0)], True)
+    s.add_static_circle((30.0, 20.0), 6.0)
+    s.add_static_polygon([(60.0, 5.0), (90.0, 5.0), (75.0, 25.0)])
+    s.set_sub_steps(3)
+    s.set_link_iterations(2)
+    s.gravity = (0.5, -9.0)
+    s.bounds = ((-1.0, -2.0), (120.0, 130.0))
+    return s
+
+
+def test_snapshot_roundtrip_on_host():
+    # bd_solver_save / bd_solver_load: every field comes back; corrupt or
+    # truncated bytes are refused
+    s = _mixed_scene()
+    data = s.save()
+    assert data[:8] == b"BD2DSNAP"
+    c = Solver.load(data)
+    assert bits_equal(state(c), state(s))
+    assert c.gravity == s.gravity and c.bounds == s.bounds
+    assert c.get_circles().tolist() == s.get_circles().tolist()
+    for getter in ("get_particle_links", "get_circle_links"):
+        for x, y in zip(getattr(c, getter)(), getattr(s, getter)()):
+            assert x.tolist() == y.tolist()
+    assert c.get_polygons_len() == 2
+    for i in range(2):
+        g, h = c.get_polygon(i), s.get_polygon(i)
+        assert g["is_static"] == h["is_static"] and bits_equal(g["particles"], h["particles"])
+    assert lib().bd_ext_get_static_collider_len(c._h) == 2
+    assert lib().bd_ext_get_sub_steps(c._h) == 3 and lib().bd_ext_get_link_iterations(c._h) == 2
+    assert c.save() == data                      # saving the loaded solver gives the same bytes
+    for bad in (data[:-1], data[:40], b"x" * len(data), data[:20] + bytes([data[20] ^ 1]) + data[21:]):
+        with pytest.raises(_lib.BendyError):
+            Solver.load(bad)

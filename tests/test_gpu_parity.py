@@ -118,6 +118,26 @@ def test_global_overlap_chain():
     run_pair(s, 6, 1.0 / 60.0, check_every=2, engine=BD_ENGINE_GLOBAL)
 
 
+def test_global_missed_pair_forces_a_redo():
+    # The row's sweep pushes its front circle about one radius forward (each
+    # pair (k, k+1) finds k already pushed, so the push grows to r).  A sentinel
+    # 1.3 beyond the front is outside both reaches at the base margin
+    # (2 * (0.5 + 0.0625) = 1.125), so the pair (front, sentinel) is not a
+    # candidate, yet the front reaches it before that pair's turn: the sweep's
+    # check must find the pair, undo the sweep and redo it with wider margins,
+    # and the result must still equal the reference's loop bit for bit.
+    s = Solver()
+    s.set_engine(BD_ENGINE_GLOBAL)
+    s.gravity = (0.0, 0.0)
+    s.bounds = ((-5000.0, -5000.0), (10000.0, 10000.0))
+    for k in range(300):
+        s.add_circle(np.array([0.5 * k, 0.0], np.float32), 0.5)
+    s.add_circle(np.array([0.5 * 299 + 1.3, 0.0], np.float32), 0.5)
+    s, o = run_pair(s, 4, 1.0 / 60.0, check_every=1, engine=BD_ENGINE_GLOBAL)
+    assert s.stats()["cc_redos"] > 0
+    assert o.get_circles()[300, 0] > 0.5 * 299 + 1.3  # the sentinel was pushed
+
+
 @pytest.mark.parametrize("engine", ENGINES)
 def test_circles_links_acc(engine):
     s = Solver()

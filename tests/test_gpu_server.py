@@ -110,3 +110,35 @@ def test_alone_equals_in_mix():
     mix.read_scene(sid2, b)
     mix.close()
     assert bits_equal(state(a), state(b))
+
+
+def test_failed_schedule_is_reported():
+    # 100 circles of radius 1 stacked within 0.05 of one point: every pair is a
+    # candidate (99 per circle), more than the SCENE engine's list capacity of
+    # 32 per circle, so the scene's schedule fails after its bounded retries.
+    # Its requests must finish with BD_ERR_COLLISION (the kernel stops solving
+    # its circle pairs, so results would not be the reference's), later
+    # requests for it too, while a scene in the same launches is unaffected.
+    from bendy2d_amd import _lib
+    bad = Solver()
+    bad.gravity = (0.0, 0.0)
+    bad.bounds = ((-1000.0, -1000.0), (2000.0, 2000.0))
+    rng = np.random.default_rng(7)
+    for _ in range(100):
+        bad.add_circle(rng.uniform(-0.05, 0.05, 2).astype(np.float32), 1.0)
+    good = scene(31)
+    srv = Server(memory_budget=1 << 30, steps_per_launch=2)
+    r_bad, sid_bad = srv.create(bad, 3, DT)
+    r_good, sid_good = srv.create(good, 5, DT)
+    assert srv.wait_idle(60)
+    r_bad2 = srv.advance(sid_bad, 2, DT)
+    assert srv.wait_idle(60)
+    done = {d["request_id"]: d for d in srv.poll()}
+    assert done[r_bad]["status"] == _lib.BD_ERR_COLLISION
+    assert done[r_bad2]["status"] == _lib.BD_ERR_COLLISION
+    assert done[r_bad]["particle_updates"] == 0
+    assert done[r_good]["status"] == 0
+    want = oracle_state(good, [(5, DT)])
+    srv.read_scene(sid_good, good)
+    assert bits_equal(state(good), want)
+    srv.close()

@@ -1,0 +1,44 @@
+"""Snapshots on the GPU: a solver saved mid-run and loaded into a fresh one
+continues exactly as the uninterrupted run (and the oracle), on every engine."""
+import pytest
+
+from helpers import bits_equal, first_diff, state, to_oracle
+from test_abi import _mixed_scene
+from bendy2d_amd import BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, Solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def scene(kind):
+    s = Solver()
+    if kind == "scene":
+        workloads.softbody(s, 777, max_side=30)
+        engine = BD_ENGINE_SCENE
+    elif kind == "global":
+        s = _mixed_scene()
+        engine = BD_ENGINE_GLOBAL
+    else:
+        workloads.cloth_large(s, n=150)
+        engine = BD_ENGINE_TILED
+    return s, engine
+
+
+@pytest.mark.parametrize("kind", ["scene", "global", "tiled"])
+def test_save_mid_run_and_continue(kind):
+    s, engine = scene(kind)
+    o = to_oracle(s, 32, candidate_mode=True)
+    dt = 1.0 / 60.0
+    for _ in range(25):
+        s.update(dt)
+        assert o.update(dt) == 0
+    assert s.engine_used == engine
+    data = s.save()          # brings the device state back and saves it
+    c = Solver.load(data)
+    for _ in range(25):
+        s.update(dt)
+        c.update(dt)
+        assert o.update(dt) == 0
+    assert c.engine_used == engine
+    a, b, w = state(s), state(c), state(o)
+    assert bits_equal(b, a), f"loaded vs uninterrupted: {first_diff(b, a)}"
+    assert bits_equal(a, w), f"uninterrupted vs oracle: {first_diff(a, w)}"

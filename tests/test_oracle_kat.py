@@ -590,6 +590,29 @@ def test_candidate_sweep_equals_all_pairs():
         assert b.cc_redos() > 0  # the margin check did trigger redos
 
 
+def test_candidate_sweep_equals_all_pairs_with_non_finite_circles():
+    # coincident circles turn NaN (normalize of a zero vector, src/circle.rs:146),
+    # and NaN or infinite circles take part in no later pair; the candidate
+    # sweep leaves them out of its grid and must still equal the all-pairs loop
+    rng = np.random.default_rng(9)
+    a = solver(g=(0.0, -9.81), bounds=((0, 0), (30, 30)))
+    b = solver(g=(0.0, -9.81), bounds=((0, 0), (30, 30)))
+    b.set_candidate_mode(True, 0.05)
+    pts = rng.uniform(1, 29, (200, 2))
+    pts[7] = pts[3]                          # coincident: NaN after the first sweep
+    pts[11] = (np.inf, 5.0)                  # infinite from the start
+    pts[12] = (np.nan, 5.0)
+    for p in pts:
+        a.add_circle(p, 0.6)
+        b.add_circle(p, 0.6)
+    for _ in range(20):
+        a.update(1 / 60)
+        b.update(1 / 60)
+        assert bits_equal(a.get_circles(), b.get_circles())
+    c = a.get_circles()
+    assert np.isnan(c[3, 0]) and np.isnan(c[7, 0]) and np.isfinite(c[20:, :4]).all()
+
+
 def test_order_of_link_sweep_matters():
     # sequential Gauss-Seidel: a chain solved forward differs from backward,
     # so a schedule that reorders dependent links would be caught
