@@ -207,6 +207,7 @@ _SIGS = [
     ("bd_scene_image_update", c_int32, [POINTER(SceneImageInfo), c_void_p, c_void_p, c_uint64, c_float, c_uint32,
                                         c_void_p]),
     ("bd_scene_image_read", c_int32, [_S, c_void_p, c_uint64]),
+    ("bd_scene_image_update_noalloc", c_int32, [POINTER(SceneImageInfo), c_void_p, c_float, c_uint32, c_void_p]),
 ]
 
 _lib = None

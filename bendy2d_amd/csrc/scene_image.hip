@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/bendy2d.h"
@@ -113,8 +114,8 @@ __global__ void k_image_begin(uint8_t *img, uint8_t *ws, uint64_t ws_cap, float 
     if (steps) w->ran = 1u;
 }
 
-// Validates the call on the host, then launches k_image_begin and the scene
-// kernel.
+// Shared by the workspace and the allocation-free update: validates the call
+// on the host, then launches k_image_begin and the scene kernel.
 int image_update(const bd_scene_image_info *info, void *image, void *ws, uint64_t ws_cap, float dt, uint32_t steps,
                  void *stream) {
     if (!info || !image) return fail(BD_ERR_ARG, "null image or info");
@@ -138,9 +139,54 @@ int image_update(const bd_scene_image_info *info, void *image, void *ws, uint64_
     return rc ? fail(rc, err) : BD_OK;
 }
 
+// Workspace of the allocation-free update: one array per device, holding the
+// circle-pair workspace of any scene an image can hold (16,384 circles need
+// 6.7 MB).  It lives until the process ends.  Host code never uses the
+// array's name as an address: scratch_address() asks the runtime for the
+// device's copy once per device.
+constexpr uint64_t kScratchBytes = 8ull << 20;
+__device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+// The current device's address of `scratch` (resolved on the first call
+// with that device current).
+int scratch_address(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(BD_ERR_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(BD_ERR_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BD_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return BD_OK;
+}
+}  // namespace
+
 }  // namespace bd
 
 extern "C" {
+
+int32_t bd_scene_image_update_noalloc(const bd_scene_image_info *info, void *image, float dt, uint32_t steps,
+                                      void *hip_stream) {
+    if (!info || !image) return BD_ERR_ARG;
+    if (info->workspace_bytes > kScratchBytes)
+        return fail(BD_ERR_CAPACITY, "the allocation-free update holds at most 8 MiB of workspace "
+                                     "(scenes of up to 16384 circles); use bd_scene_image_update with a workspace");
+    void *ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return image_update(info, image, ws, kScratchBytes, dt, steps, hip_stream);
+}
 
 int32_t bd_scene_image_describe(const bd_solver *s, bd_scene_image_info *info) {
     if (!s || !info) return BD_ERR_ARG;
@@ -154,6 +200,9 @@ int32_t bd_scene_image_describe(const bd_solver *s, bd_scene_image_info *info) {
     if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {
         rc = scene_kernels_init(err);
         if (rc) return fail(rc, err);
+        void *ws = nullptr;
+        rc = scratch_address(&ws);  // the allocation-free update's workspace on this device
+        if (rc) return rc;
     }
     std::memset(info, 0, sizeof(*info));
     info->image_bytes = kImgChunk + img.init_bytes;

@@ -448,6 +448,17 @@ int32_t bd_scene_image_update(const bd_scene_image_info *info, void *image, void
  * same scene (e.g. the one described).  BD_ERR_COLLISION if the scene's
  * circle-pair schedule failed. */
 int32_t bd_scene_image_read(bd_solver *s, const void *host_image, uint64_t bytes);
+/* bd_scene_image_update without a workspace argument.  No counterpart in the
+ * reference: for callers that cannot allocate device memory at all (code
+ * under graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is the library's static per-device array of 8 MiB, enough for
+ * every scene an image holds (16,384 circles need 6.7 MB); larger
+ * workspace_bytes are refused with BD_ERR_CAPACITY.  It allocates nothing.
+ * Every call on one device shares that device's array, so calls that overlap
+ * on one device (concurrent streams or threads) race; calls in sequence on
+ * one stream do not, since no call relies on what an earlier one left there. */
+int32_t bd_scene_image_update_noalloc(const bd_scene_image_info *info, void *image, float dt, uint32_t steps,
+                                      void *hip_stream);
 
 /* ------------------------------------------------------------------ */
 /* Scene generators (synthetic inputs of the benchmark workloads)       */
