@@ -168,6 +168,43 @@ def bench_cloth_small(steps=1000):
             "engine": s.engine_used, "bodies": N, "updates": steps}
 
 
+def bench_readback():
+    """Frame time of update + reading every particle back (the reference's
+    callers read positions each frame, src/solver.rs:86-104), with the
+    readback path on (the copy queued behind each update on a second stream)
+    and off (a synchronous copy when read): the 32x32 pinned cloth (100
+    frames) and the 1024x1024 cloth (10 frames)."""
+    from bendy2d_amd import Solver, workloads
+    out = {}
+    for name, build, frames in (("cloth_32", lambda s: workloads.cloth_pinned(s), 100),
+                                ("cloth_1024", lambda s: workloads.cloth_large(s), 10)):
+        row = {}
+        for mode in ("off", "on"):
+            s = Solver()
+            dt, _ = build(s)
+            s.set_readback(mode == "on")
+            s.update(dt)
+            s.get_particles()
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                s.update(dt)
+                s.get_particles()
+            row[f"ms_per_frame_readback_{mode}"] = (time.perf_counter() - t0) * 1e3 / frames
+            s.close()
+        s = Solver()
+        dt, _ = build(s)
+        s.update(dt)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            s.update(dt)
+        s.sync()
+        row["ms_per_frame_no_read"] = (time.perf_counter() - t0) * 1e3 / frames
+        s.close()
+        out[name] = row
+    return out
+
+
 def bench_serving(seconds, rate, launch_cfg, steps_per_launch=64, budget=64 << 30, seed=4, drain_limit=30.0,
                   launch_budget_us=20000):
     """Poisson stream of requests: half create a new softbody scene (generated
@@ -444,6 +481,8 @@ def main():
     ap.add_argument("--dense-at", type=int, default=300,
                     help="also time 10 updates of the batch from this update on (0 = skip)")
     ap.add_argument("--launch", type=str, default="", help="batch launch config as JSON (skips the autotuner)")
+    ap.add_argument("--no-window100", dest="window100", action="store_false",
+                    help="skip timing updates 1-100 of a second copy of the batch")
     ap.add_argument("--only", choices=("all", "batched", "cloth", "copy", "gas", "cloth32"), default="all",
                     help="run one part alone (for profiler runs); only 'all' prints the JSON line")
     ap.add_argument("--cpu-leg", type=int, default=None, help=argparse.SUPPRESS)
@@ -487,26 +526,70 @@ def main():
     log(f"copy calibration: {copy}")
     if args.only == "batched":
         return
+    b = bat["batch"]
+    done = args.warmup + bat["steps"]
 
-    # the same batch later on, once the large softbodies have collapsed into
-    # piles in dense contact (reported beside the headline, not in it)
+    # the plain path (bd_update per scene, the drop-in) on a sample of the
+    # scenes in the state the batch has reached (every k-th scene)
+    k_sample = 1 if not args.plain_sample else max(1, len(scenes) // args.plain_sample)
+    sample_ids = list(range(0, len(scenes), k_sample))
+
+    def plain_at_batch_state():
+        from bendy2d_amd import Solver
+        b.sync()
+        sample = []
+        for i in sample_ids:
+            c = scenes[i].clone()
+            b.read_scene(i, c)
+            sample.append(c)
+        spts = sum(c.stats()["n_points"] for c in sample)
+        r = bench_plain_per_scene(sample, spts)
+        for c in sample:
+            c.close()
+        return r, spts
+
+    plain, _ = plain_at_batch_state()
+    plain_update = done + 2
+    log(f"plain per-scene (update {plain_update}): {plain['pu_per_s']:.3e} PU/s over {len(sample_ids)} scenes")
+
+    # the error window: updates 1..100 of the batch, timed whole on a copy of
+    # the scenes from rest (the batch itself continues from where it is)
+    window100 = None
+    if args.window100:
+        from bendy2d_amd import Batch
+        w = Batch()
+        for s in scenes:
+            w.add_scene(s)
+        w.config = bat["config"]
+        w.update(DT, 0)
+        w.sync()
+        t0 = time.perf_counter()
+        w.update(DT, 100)
+        w.sync()
+        el = time.perf_counter() - t0
+        window100 = {"updates": "1-100", "pu_per_s": pts * 8 * 100 / el, "ms_per_update": el * 1e3 / 100,
+                     "cc": w.cc_stats()}
+        w.close()
+        log(f"window 1-100: {window100}")
+
+    # the same batch once the large softbodies have collapsed into piles in
+    # dense contact, with the plain path on the same states
     dense = None
-    if args.dense_at > bat["steps"] + args.warmup:
-        b = bat["batch"]
-        b.update(DT, args.dense_at - bat["steps"] - args.warmup)
+    if args.dense_at > done:
+        b.update(DT, args.dense_at - done)
         b.sync()
         t0 = time.perf_counter()
         b.update(DT, 10)
         b.sync()
         el = time.perf_counter() - t0
+        done = args.dense_at + 10
         dense = {"updates": f"{args.dense_at + 1}-{args.dense_at + 10}", "pu_per_s": pts * 8 * 10 / el,
                  "ms_per_update": el * 1e3 / 10, "cc": b.cc_stats()}
+        dplain, _ = plain_at_batch_state()
+        dense["plain_per_scene"] = {"value": dplain["pu_per_s"], "scenes": len(sample_ids),
+                                    "update": args.dense_at + 12}
+        dense["batched_vs_plain"] = dense["pu_per_s"] / dplain["pu_per_s"]
         log(f"dense regime: {dense}")
-
-    sample = scenes if not args.plain_sample else scenes[::max(1, len(scenes) // args.plain_sample)]
-    spts = sum(s.stats()["n_points"] for s in sample)
-    plain = bench_plain_per_scene(sample, spts)
-    log(f"plain per-scene: {plain['pu_per_s']:.3e} PU/s over {len(sample)} scenes")
 
     cloth = bench_cloth(args.cloth, 10, 2) if args.cloth else None
     if cloth:
@@ -517,11 +600,14 @@ def main():
     cloth32 = bench_cloth_small() if args.gas_steps else None
     if cloth32:
         log(f"cloth 32^2: {cloth32}")
+    readback = bench_readback() if args.gas_steps else None
+    if readback:
+        log(f"readback: {readback}")
 
     serving = None
     if args.serve_seconds > 0 or args.serve_rate_sustained > 0:
         # free the batched workload's device memory first
-        bat["batch"].close()
+        b.close()
         for s in scenes:
             s.close()
     if args.serve_seconds > 0:
@@ -550,15 +636,18 @@ def main():
         if cloth32:
             cloth32["cpu_baseline"] = cpus["cloth_32"]
 
-    # dominant kernel: one launch (graph replay) of k_scenes, both size classes
-    # running concurrently on their streams, timed by HIP events on the batch's
-    # stream around all timed launches
+    # dominant kernel: one launch (graph replay) of k_scenes, its size classes
+    # running concurrently on their streams, timed by HIP events on the
+    # batch's stream around all timed launches.  `achieved` is the HBM traffic
+    # rocprofv3 measured per launch over the same window (FETCH_SIZE +
+    # WRITE_SIZE, scaled by a copy kernel); the algorithmic model's bytes,
+    # which charge HBM for state that stays in LDS, are reported beside it.
     per_launch_s = bat["device_ms"] / 1e3 / max(bat["replays"], 1)
-    bytes_per_launch = bat["alg_bytes"] / max(bat["replays"], 1)
-    achieved = bytes_per_launch / per_launch_s
+    alg_per_launch = bat["alg_bytes"] / max(bat["replays"], 1)
     spl = int(bat["config"]["steps_per_launch"])
     tpu = profiled_traffic_per_update() if args.scenes == 4096 else None
     traffic = tpu * spl if tpu else None
+    achieved = traffic / per_launch_s if traffic else None
     out = {
         "metric": "particle-updates/s batched vs plain (1 GPU) + p50/p99 request latency",
         "value": bat["pu_per_s"],
@@ -576,28 +665,36 @@ def main():
             "workload": f"{args.scenes} independent softbody scenes (grid of circles r=0.35, side log-uniform "
                         "4..128, structural+shear circle links, 1-8 static circle/convex-polygon colliders, "
                         "circle-pair collision on), 8 substeps and 1 link sweep per update, batched persistent "
-                        "launch replayed as a hipGraph; updates 1.." + str(args.warmup + bat["steps"]) + " from rest",
+                        f"launch replayed as a hipGraph; timed: updates {args.warmup + 1}-{args.warmup + bat['steps']} "
+                        "from rest (the softbodies falling, the first contacts from about update 90); updates 1-100 "
+                        "and the dense-contact regime (updates 301-310) are in window_1_100 and dense_regime",
             "scenes": args.scenes, "bodies": pts, "links": links, "substeps": 8,
             "launch": {k: v for k, v in bat["config"].items()},
         },
         "roofline": {
-            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "bound": "hbm", "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": traffic,
             "traffic_source": "profiles/k_scenes_batched_softbody4096.md: rocprofv3 FETCH_SIZE + WRITE_SIZE per "
-                              "update, scaled by a 1 GiB copy kernel, times this run's updates per launch",
+                              "update over the same window, scaled by a 1 GiB copy kernel, times this run's "
+                              "updates per launch",
+            "algorithmic_bytes_per_launch": alg_per_launch,
+            "algorithmic_GBps": alg_per_launch / per_launch_s / 1e9,
+            "algorithmic_frac": alg_per_launch / per_launch_s / HBM_PEAK,
             "measured_copy_GBps": copy["copy_GBps"],
-            "kernel": "k_scenes, one launch = both size classes concurrently, "
-                      f"{bat['config']['steps_per_launch']} updates; {bytes_per_launch / 1e9:.3f} GB algorithmic "
-                      f"per launch in {per_launch_s * 1e3:.3f} ms",
+            "kernel": f"k_scenes, one launch = its size classes concurrently, {spl} updates, "
+                      f"{per_launch_s * 1e3:.3f} ms per launch",
         },
         "cpu_baseline": cpu,
-        "plain_per_scene": {"value": plain["pu_per_s"], "scenes": len(sample),
-                            "path": "bd_update per scene (drop-in), AUTO engine"},
-        "batched_vs_plain": bat["pu_per_s"] / plain["pu_per_s"],
+        "plain_per_scene": {"value": plain["pu_per_s"], "scenes": len(sample_ids), "update": plain_update,
+                            "path": "bd_update per scene (drop-in), AUTO engine, on the batch's state after the "
+                                    "timed window"} if plain else None,
+        "batched_vs_plain": bat["pu_per_s"] / plain["pu_per_s"] if plain else None,
+        "window_1_100": window100,
         "dense_regime": dense,
         "cloth_1024": cloth,
         "gas_262144": gas,
         "cloth_32": cloth32,
+        "readback": readback,
         "serving": serving,
         "serving_sustained_load": serving_sustained,
     }

@@ -164,6 +164,7 @@ _SIGS = [
     ("bd_ext_set_tiling", c_int32, [_S, c_uint32, c_uint32]),
     ("bd_ext_tiled_selfcheck", c_int64, [_S, c_uint32, c_uint32, POINTER(c_uint32)]),
     ("bd_ext_sync", c_int32, [_S]),
+    ("bd_ext_set_readback", c_int32, [_S, c_int32]),
     ("bd_ext_get_stats", c_int32, [_S, POINTER(Stats)]),
     ("bd_ext_get_static_collider", c_int32, [_S, c_uint64, POINTER(Vec2), POINTER(c_float), c_void_p, c_uint64,
                                               POINTER(c_uint64)]),

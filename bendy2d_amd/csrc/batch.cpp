@@ -301,7 +301,7 @@ int64_t bd_batch_add_scene(bd_batch *b, const bd_solver *scene) {
     Plan p;
     rc = build_plan(h, p, err);
     if (rc) return fail(rc, err);
-    if (!h.polys.empty() || p.N > 16384) return fail(BD_ERR_CAPACITY, "batch scenes hold no soft polygons and at most 16384 bodies");
+    if (p.N > 16384) return fail(BD_ERR_CAPACITY, "batch scenes hold at most 16384 bodies");
     SceneImage img;
     rc = build_scene_image(h, p, img, err);
     if (rc) return fail(rc, err);
@@ -372,25 +372,16 @@ int32_t bd_batch_read_scene(bd_batch *b, int64_t id, bd_solver *out) {
         if (rc) return fail(rc, err);
     }
     const SceneDesc &d = b->core.descs[id];
-    std::vector<float2> pos(d.N), prev(d.N);
     SceneState st;
     const uint8_t *chunk = b->arena.p + b->chunk_off[id];
-    if (d.N) {
-        if (hipMemcpyAsync(pos.data(), chunk + d.pos_off, 8ull * d.N, hipMemcpyDeviceToHost, b->core.stream) != hipSuccess ||
-            hipMemcpyAsync(prev.data(), chunk + d.prev_off, 8ull * d.N, hipMemcpyDeviceToHost, b->core.stream) != hipSuccess)
-            return fail(BD_ERR_DEVICE, "scene readback failed");
-    }
-    if (hipMemcpyAsync(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost, b->core.stream) != hipSuccess ||
+    std::vector<uint8_t> buf(b->images[id].init_bytes);
+    if ((!buf.empty() &&
+         hipMemcpyAsync(buf.data(), chunk, buf.size(), hipMemcpyDeviceToHost, b->core.stream) != hipSuccess) ||
+        hipMemcpyAsync(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost, b->core.stream) != hipSuccess ||
         hipStreamSynchronize(b->core.stream) != hipSuccess)
         return fail(BD_ERR_DEVICE, "scene readback failed");
     HostScene h;
-    h.p_pos.resize(d.Np); h.p_prev.resize(d.Np);
-    h.c_pos.resize(d.Nc); h.c_prev.resize(d.Nc);
-    for (uint32_t i = 0; i < d.N; i++) {
-        V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
-        if (i < d.Np) { h.p_pos[i] = p; h.p_prev[i] = q; }
-        else { h.c_pos[i - d.Np] = p; h.c_prev[i - d.Np] = q; }
-    }
+    scene_chunk_to_host(d, buf.data(), h);
     int rc = solver_import_state(out, h, b->updates_run[id], err);
     if (rc) return fail(rc, err);
     if (st.err) return fail(BD_ERR_COLLISION, "circle pair schedule failed for this scene");

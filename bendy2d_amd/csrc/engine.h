@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/bendy2d.h"
 #include "scene.h"
 
 namespace bd {
@@ -40,7 +41,24 @@ public:
     // Run one update (all substeps).
     virtual int update(const StepParams &sp, Stats &st, std::string &err) = 0;
     // Copy positions back into the host scene (and static polygon centres).
-    virtual int download(HostScene &s, std::string &err) = 0;
+    virtual int download(HostScene &s, std::string &err) {
+        std::vector<float2> buf(state_count());
+        int rc = copy_state(buf.data(), hipMemcpyDeviceToHost, stream(), err);
+        if (rc) return rc;
+        if (hipStreamSynchronize(stream()) != hipSuccess) {
+            err = "hipStreamSynchronize failed";
+            return BD_ERR_DEVICE;
+        }
+        return unpack(buf.data(), s, err);
+    }
+    // What a download reads, as state_count() float2: pos[N], prev[N], then
+    // the engine's extra words (polygon centres, the circle-pair state);
+    // copy_state queues its copy into dst (device or host memory, by kind) on
+    // stream, unpack fills the host scene from a host copy of it.
+    virtual size_t state_count() const = 0;
+    virtual int copy_state(void *dst, hipMemcpyKind kind, hipStream_t stream, std::string &err) = 0;
+    virtual int unpack(const float2 *src, HostScene &s, std::string &err) = 0;
+    virtual hipStream_t stream() const = 0;
     virtual int sync(std::string &err) = 0;
     virtual int kind() const = 0;
 };

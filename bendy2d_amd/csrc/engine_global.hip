@@ -68,55 +68,6 @@ __global__ void k_poly_centers(const float2 *__restrict__ pos, const uint2 *__re
     centers[p] = make_float2(sx / n, sy / n);
 }
 
-// common::line_intersection (src/common.rs:4-26)
-__device__ __forceinline__ bool line_intersection(float2 p1, float2 p2, float2 p3, float2 p4, float2 &out) {
-    float s1x = p2.x - p1.x, s1y = p2.y - p1.y;
-    float s2x = p4.x - p3.x, s2y = p4.y - p3.y;
-    float den = (-s2x) * s1y + s1x * s2y;
-    float s = ((-s1y) * (p1.x - p3.x) + s1x * (p1.y - p3.y)) / den;
-    float t = (s2x * (p1.y - p3.y) - s2y * (p1.x - p3.x)) / den;
-    if (s >= 0.0f && s <= 1.0f && t >= 0.0f && t <= 1.0f) {
-        out = make_float2(p1.x + (t * s1x), p1.y + (t * s1y));
-        return true;
-    }
-    return false;
-}
-
-__device__ __forceinline__ float2 f2sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 f2add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 f2mul(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-__device__ __forceinline__ float f2dot(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
-__device__ __forceinline__ float2 f2normalize(float2 a) {
-    float n = sqrtf(f2dot(a, a));
-    return make_float2(a.x / n, a.y / n);
-}
-__device__ __forceinline__ float f2mag(float2 a) { return sqrtf(f2dot(a, a)); }
-
-// Polygon::resolve_line_intersection (src/polygon.rs:164-216)
-__device__ bool resolve_line(float2 self_center, float2 pa, float2 pb, float2 op, float2 other_center,
-                             float2 &na, float2 &nb, float2 &no) {
-    float2 inter;
-    if (!line_intersection(pa, pb, op, other_center, inter)) return false;
-    float2 nl = f2normalize(f2sub(pb, pa));                                    // :175
-    float2 cp = f2mul(nl, f2dot(nl, f2sub(self_center, inter)) / f2dot(nl, nl));  // :177-179
-    float2 nin = f2normalize(f2sub(self_center, f2add(inter, cp)));           // :181
-    float da = f2mag(f2sub(inter, pa));                                       // :183
-    float db = f2mag(f2sub(inter, pb));                                       // :184
-    float dab = da + db;                                                      // :186
-    float ia = db / dab, ib = da / dab;                                       // :188-189
-    float2 diff = f2sub(inter, op);                                           // :191
-    float2 ion = f2mul(nin, f2dot(nin, diff) / f2dot(nin, nin));             // :193-194
-    float2 third = make_float2(ion.x / 3.0f, ion.y / 3.0f);                   // :196
-    float2 line = f2mul(third, 2.0f);                                         // :198
-    float2 da_v = f2mul(line, ia), db_v = f2mul(line, ib);                    // :200-201
-    float2 ra = f2sub(pa, da_v), rb = f2sub(pb, db_v);                        // :203-204
-    float2 ray_end = f2sub(op, f2mul(nin, 10000.0f));                         // :208
-    float2 np;
-    if (!line_intersection(pa, pb, op, ray_end, np)) return false;            // :206-213
-    na = ra; nb = rb; no = np;
-    return true;
-}
-
 // Polygon pair loop (src/solver.rs:178-187 -> Polygon::solve_polygon
 // src/polygon.rs:142-162) in one workgroup.  Edges run in order; within an
 // edge the other polygon's points are independent (each reads only the stale
@@ -1202,26 +1153,32 @@ public:
         return BD_OK;
     }
 
-    int download(HostScene &s, std::string &err) override {
-        if (!host_shape_matches(s, Np_, Nc_, Ng_, err)) return BD_ERR_STATE;
+    size_t state_count() const override { return 2ull * N_ + P_; }
+    hipStream_t stream() const override { return stream_; }
+
+    int copy_state(void *dst, hipMemcpyKind kind, hipStream_t st, std::string &err) override {
+        float2 *d = (float2 *)dst;
         if (N_) {
-            std::vector<float2> pos(N_), prev(N_);
-            BD_HIP(hipMemcpyAsync(pos.data(), pos_.p, N_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));
-            BD_HIP(hipMemcpyAsync(prev.data(), prev_.p, N_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));
-            std::vector<float2> centers(P_);
-            if (P_) BD_HIP(hipMemcpyAsync(centers.data(), centers_.p, P_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));
-            BD_HIP(hipStreamSynchronize(stream_));
-            auto get = [&](std::vector<V2> &a, std::vector<V2> &b, uint32_t base) {
-                for (size_t i = 0; i < a.size(); i++) {
-                    a[i] = V2{pos[base + i].x, pos[base + i].y};
-                    b[i] = V2{prev[base + i].x, prev[base + i].y};
-                }
-            };
-            get(s.p_pos, s.p_prev, 0);
-            get(s.c_pos, s.c_prev, Np_);
-            get(s.g_pos, s.g_prev, Np_ + Nc_);
-            for (uint32_t p : static_poly_) s.polys[p].center = V2{centers[p].x, centers[p].y};
+            BD_HIP(hipMemcpyAsync(d, pos_.p, N_ * sizeof(float2), kind, st));
+            BD_HIP(hipMemcpyAsync(d + N_, prev_.p, N_ * sizeof(float2), kind, st));
         }
+        if (P_) BD_HIP(hipMemcpyAsync(d + 2 * N_, centers_.p, P_ * sizeof(float2), kind, st));
+        return BD_OK;
+    }
+
+    int unpack(const float2 *src, HostScene &s, std::string &err) override {
+        if (!host_shape_matches(s, Np_, Nc_, Ng_, err)) return BD_ERR_STATE;
+        const float2 *pos = src, *prev = src + N_, *centers = src + 2 * N_;
+        auto get = [&](std::vector<V2> &a, std::vector<V2> &b, uint32_t base) {
+            for (size_t i = 0; i < a.size(); i++) {
+                a[i] = V2{pos[base + i].x, pos[base + i].y};
+                b[i] = V2{prev[base + i].x, prev[base + i].y};
+            }
+        };
+        get(s.p_pos, s.p_prev, 0);
+        get(s.c_pos, s.c_prev, Np_);
+        get(s.g_pos, s.g_prev, Np_ + Nc_);
+        for (uint32_t p : static_poly_) s.polys[p].center = V2{centers[p].x, centers[p].y};
         return BD_OK;
     }
 

@@ -151,6 +151,15 @@ struct CCShared {
     uint32_t halt;           // the scene's time budget for this launch is spent
     float red9[9 * 16];      // workgroup drift summary (see drift_summary)
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
+    uint32_t poly_last[2];   // polygon pairs: 1 + the last point of the other polygon that
+                             // hit the current edge, two slots used alternately
+    // statistics of the scene during a launch, written by lane 0 only (in LDS
+    // rather than in every lane's registers)
+    struct Counts {
+        uint32_t rounds, redos, rebuilds, kept, unchecked, n_out, n_far, max_list;
+        float smax;
+        uint64_t t_build, t_count, t_lists, t_sort, t_links, t_sweep, t_bodies;
+    } cnt;
 };
 
 // Drift of the circles from their list-build positions, in one pass: the sum
@@ -418,10 +427,16 @@ __device__ __forceinline__ bool missed_pair_touches(const CCWork &w, const float
 // list-build position, relative to the scene shift, over the whole sweep).
 // A pair whose list-build positions are more than r_c + r_x + v_c + v_x apart
 // stayed apart; the rest get the exact test above.
-template <int WG>
-__device__ int overruns_check(const CCWork &w, const float *crad, const float *vb, const uint16_t *over,
+//
+// With few overruns (at most kOutCap, listed in LDS) pairs of two overrun
+// circles are tested all against all and the grid scan skips them; with more
+// (listed in the workspace) the grid scan covers them too, vmax then being
+// the largest bound of any circle, and a pair of two overruns is tested from
+// both ends.
+template <int WG, typename Idx>
+__device__ int overruns_check(const CCWork &w, const float *crad, const float *vb, const Idx *over,
                               uint32_t n_over, float vmax, float rmax, double h, double inv_h, uint32_t tmask,
-                              float *s_red) {
+                              float *s_red, bool scan_overruns) {
     // rounding margin: a fifth of the overrun circle's skin (the margin the
     // skin test leaves too) plus about 80 ulps of the coordinates
     auto margin = [&](float2 p, float sk) { return 0.2f * sk + 1e-5f * (fabsf(p.x) + fabsf(p.y) + 1.0f); };
@@ -444,13 +459,18 @@ __device__ int overruns_check(const CCWork &w, const float *crad, const float *v
             for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
                 const float4 r = w.rec[s];
                 const uint32_t x = __float_as_uint(r.w) & ~kOutlierBit;
-                const float vx = vb[x];
-                if (x == c || !(vx >= 0.0f)) continue;  // itself, or beyond its skin: below
+                float vx = vb[x];
+                if (x == c) continue;
+                if (!(vx >= 0.0f)) {  // beyond its skin: below, or here when overruns are scanned
+                    if (!scan_overruns) continue;
+                    vx = -vx;
+                }
                 if (apart(p, make_float2(r.x, r.y), ((rc + crad[x]) + (vc + vx)) + mc)) continue;
                 if (missed_pair_touches(w, crad, c, x)) bad = 1.0f;
             }
         }
     }
+    if (!scan_overruns)
     for (uint32_t q = threadIdx.x; q < n_over * n_over; q += WG) {
         const uint32_t a = q / n_over, b = q - a * n_over;
         if (a >= b) continue;
@@ -465,13 +485,12 @@ __device__ int overruns_check(const CCWork &w, const float *crad, const float *v
     return (int)wg_max<WG>(fmaxf(bad, 2.0f * far), s_red);
 }
 
-// Per-scene counters of the circle-pair schedule, kept in registers during a
-// launch (uniform across the workgroup).
+// Per-scene state of the circle-pair schedule that steers it, kept in
+// registers during a launch (uniform across the workgroup); its statistics
+// are in CCShared::cnt.
 struct CCStats {
-    uint32_t rounds, redos, rebuilds, pairs, since_build, kept, unchecked;
-    uint32_t n_out, n_far, max_list;
-    float s_cell, smax, smin;
-    uint64_t t_build, t_count, t_lists, t_sort;
+    uint32_t pairs, since_build;
+    float s_cell, smin;
 };
 
 // One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
@@ -497,7 +516,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
     for (int attempt = 0;; attempt++) {
-        if (attempt > 12) return 2u;
+        if (attempt > 20) return 2u;
         bool rebuild = !list_valid;
         float2 u = make_float2(0.0f, 0.0f);
         if (!rebuild) {
@@ -529,7 +548,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             }
         }
         if (rebuild) {
-            cs.rebuilds++;
+            if (tid == 0) sh->cnt.rebuilds++;
             const uint64_t tb0 = wall_clock64();
             // per-circle skins: the base, or what the circle's recent sweeps pushed it;
             // s_cell: the skin at most kOutCap circles exceed (log2 histogram)
@@ -607,10 +626,17 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             wg_scan_global<WG>(w.off, Nc + 1, s_u32);
             const uint32_t total = uni(w.off[Nc]);
             if (total > kCandCap * Nc) {
-                // more candidates than the workspace holds: smaller skins
+                // more candidates than the workspace holds: first the outlying
+                // skins come down to the outlier threshold (they list the most
+                // partners), and only with no outliers left the base skin
+                // shrinks; the skins the other circles showed they need stay
                 __syncthreads();
-                sbase = 0.7f * sbase;
-                for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = 0.5f * w.need[c];
+                if (n_out) {
+                    for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = fminf(w.need[c], 0.5f * s_cell);
+                } else {
+                    sbase = 0.7f * sbase;
+                    for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = fminf(w.need[c], sbase);
+                }
                 __syncthreads();
                 list_valid = false;
                 if (sbase < 1e-4f * fmaxf(rmax, 1e-3f)) return 1u;
@@ -651,7 +677,9 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 if (n <= 1u) continue;
                 if (n <= 8u) sort_in_registers<8>(my, n);
                 else if (n <= 16u) sort_in_registers<16>(my, n);
-                else if (n <= 32u) sort_in_registers<32>(my, n);
+                // 32 registers more than a 1,024-lane workgroup's 128 allow
+                // without spilling: there such lists go to a wavefront
+                else if (WG < 1024 && n <= 32u) sort_in_registers<(WG < 1024 ? 32 : 16)>(my, n);
                 else if (n <= kMidList) w.wl[atomicAdd(&sh->nmid, 1u)] = c;
                 else w.wl[wl_end - atomicAdd(&sh->nlong, 1u)] = c;
             }
@@ -682,17 +710,20 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             cs.since_build = 0;
             list_valid = true;
             cs.pairs = total / 2;
-            cs.n_out = n_out;
-            cs.n_far = n_far;
             cs.s_cell = s_cell;
-            cs.smax = smax;
             u = make_float2(0.0f, 0.0f);
-            cs.max_list = (uint32_t)wg_max<WG>((float)longest, s_red);
+            const uint32_t max_list = (uint32_t)wg_max<WG>((float)longest, s_red);
             const uint64_t tb3 = wall_clock64();
-            cs.t_build += tb3 - tb0;
-            cs.t_count += tb1 - tb0;
-            cs.t_lists += tb2 - tb1;
-            cs.t_sort += tb3 - tb2;
+            if (tid == 0) {
+                sh->cnt.n_out = n_out;
+                sh->cnt.n_far = n_far;
+                sh->cnt.smax = smax;
+                sh->cnt.max_list = max_list;
+                sh->cnt.t_build += tb3 - tb0;
+                sh->cnt.t_count += tb1 - tb0;
+                sh->cnt.t_lists += tb2 - tb1;
+                sh->cnt.t_sort += tb3 - tb2;
+            }
         }
         // ---- sweep: worklist rounds.  nxp (in key, free outside list builds):
         // each circle's next unprocessed partner, kNone when its list is done,
@@ -770,13 +801,15 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             }
             __syncthreads();
             n = uni(sh->cnt3[(r + 1) % 3]);
-            cs.rounds++;
+            if (tid == 0) sh->cnt.rounds++;
         }
         // every list must be exhausted (a schedule that stalls is an error);
         // circles beyond 0.8 of their skin go to out[] (free after list builds)
         // and vb (stamp, free after the rounds) gets each circle's bound v,
         // negated for those
-        float left = 0.0f, vmax = 0.0f;
+        // (the worklist array, free after the rounds, lists every overrun
+        // circle when there are more than kOutCap)
+        float left = 0.0f, vmax = 0.0f, vall = 0.0f;
         float *vb = (float *)w.stamp;
         if (tid == 0) sh->nout = 0u;
         __syncthreads();
@@ -791,28 +824,33 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 vb[c] = v;
             } else {
                 vb[c] = -v;
+                vall = fmaxf(vall, v);
                 uint32_t k = atomicAdd(&sh->nout, 1u);
                 if (k < kOutCap) sh->out[k] = (uint16_t)c;
+                w.wl[k] = c;
             }
         }
         if (wg_max<WG>(left, s_red) > 0.0f) return 3u;
         const uint32_t n_over = uni(sh->nout);
         vmax = wg_max<WG>(vmax, s_red);
         const double h = cell_size(rmax, cs.s_cell);  // the grid of the list build
-        const int check = n_over == 0u ? 0
-                          : n_over > kOutCap ? 2
-                                             : overruns_check<WG>(w, crad, vb, sh->out, n_over, vmax, rmax, h, 1.0 / h,
-                                                                  tmask, s_red);
+        int check = 0;
+        if (n_over > kOutCap) {
+            vall = fmaxf(wg_max<WG>(vall, s_red), vmax);
+            check = overruns_check<WG>(w, crad, vb, w.wl, n_over, vall, rmax, h, 1.0 / h, tmask, s_red, true);
+        } else if (n_over) {
+            check = overruns_check<WG>(w, crad, vb, sh->out, n_over, vmax, rmax, h, 1.0 / h, tmask, s_red, false);
+        }
         if (check == 0) {
             cs.since_build++;
-            if (n_over) cs.kept++;
+            if (n_over && tid == 0) sh->cnt.kept++;
             return 0u;
         }
-        if (check == 2) cs.unchecked++;
+        if (check == 2 && tid == 0) sh->cnt.unchecked++;
         // undo and rebuild with the skins the circles needed
         for (uint32_t c = tid; c < Nc; c += WG) P[Np + c] = w.save[c];
         __syncthreads();
-        cs.redos++;
+        if (tid == 0) sh->cnt.redos++;
         list_valid = false;
     }
 }
@@ -837,7 +875,10 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     float2 *P, *Q;
     if constexpr (MODE >= 1) P = s_lds; else P = gpos;
     if constexpr (MODE == 2) Q = s_lds + d.N; else Q = gprev;
-    const uint32_t N = d.N, Np = d.Np, Nc = d.Nc;
+    const uint32_t N = d.N, Np = d.Np, Nc = d.Nc, NpNc = d.Np + d.Nc, npoly = d.P;
+    const uint4 *ptab = (const uint4 *)(base + d.poly_off);
+    float2 *ctr = (float2 *)(base + d.ctr_off);
+    const uint32_t *gstatic = (const uint32_t *)(base + d.gst_off);
     const int tid = threadIdx.x;
     // circle radius: the descriptor's common one when all circles share it
     const bool uni_r = (d.flags & kFlagUniformR) != 0;
@@ -891,9 +932,13 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     CCStats cs{};
     cs.pairs = uni(st->pairs);
     cs.since_build = uni(st->since_build);
-    cs.n_out = uni(st->n_out); cs.n_far = uni(st->n_far); cs.max_list = uni(st->max_list);
-    cs.s_cell = uni(st->s_cell); cs.smax = uni(st->smax); cs.smin = uni(st->smin);
-    uint64_t t_links = 0, t_sweep = 0, t_bodies = 0;
+    cs.s_cell = uni(st->s_cell); cs.smin = uni(st->smin);
+    if (tid == 0) {
+        CCShared::Counts &k = s_cc->cnt;
+        k = CCShared::Counts{};
+        k.n_out = st->n_out; k.n_far = st->n_far; k.max_list = st->max_list; k.smax = st->smax;
+    }
+    __syncthreads();
     bool acc = (d.flags & kFlagAcc) != 0;
     const float2 *acc0 = (const float2 *)(base + d.acc0_off);
 
@@ -903,8 +948,26 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         for (uint32_t sub = 0; sub < d.sub_steps; sub++) {
             // ---- links: dependency layers, k sweeps
             const uint64_t tl0 = wall_clock64();
+            // polygon centres (Polygon::calc_center, src/polygon.rs:231-237) as
+            // Polygon::solve_links computes them before the polygon's links in
+            // the last sweep; no other link moves a polygon's points, so the
+            // start of that sweep gives the same sums
+            auto poly_centres = [&]() {
+                for (uint32_t q = tid; q < npoly; q += WG) {
+                    const uint4 r = ptab[q];
+                    float sx = 0.0f, sy = 0.0f;
+                    for (uint32_t i = 0; i < r.y; i++) {
+                        const float2 v = P[r.x + i];
+                        sx = sx + v.x; sy = sy + v.y;
+                    }
+                    const float n = (float)r.y;
+                    ctr[q] = make_float2(sx / n, sy / n);
+                }
+                __syncthreads();
+            };
             auto link_sweeps = [&](const LinkS *lk) {
             for (uint32_t it = 0; it < d.link_iters; it++) {
+                if (npoly && it + 1 == d.link_iters) poly_centres();
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = loff[L], hi = loff[L + 1];
                     // not unrolled (measured faster than unrolling by 2, 4 or 8:
@@ -919,7 +982,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                         if (l + WG < hi) nxt = lk[l + WG];
                         uint32_t a = rec.a & 0x7fffu, b = rec.b & 0x7fffu;
                         float2 A = P[a], B = P[b];
-                        if (a >= Np) {
+                        if (a >= Np && a < NpNc) {
                             if (uni_r) solve_circle_link_w(A.x, A.y, B.x, B.y, rec.t, r2u, r2u, scaleu);
                             else solve_circle_link(A.x, A.y, B.x, B.y, rec.t, crad[a - Np], crad[b - Np]);
                         } else {
@@ -935,14 +998,53 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             };
             if (links_lds) link_sweeps(s_links);
             else link_sweeps(links);
-            t_links += wall_clock64() - tl0;
+            if (npoly && d.link_iters == 0) poly_centres();
+            if (tid == 0) s_cc->cnt.t_links += wall_clock64() - tl0;
             // ---- circle pairs
             if (Nc >= 2 && !err)
             {
                 const uint64_t t0 = wall_clock64();
                 err = circle_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl, skin,
                                        list_valid, cs);
-                t_sweep += wall_clock64() - t0;
+                if (tid == 0) s_cc->cnt.t_sweep += wall_clock64() - t0;
+            }
+            // ---- polygon pairs (src/solver.rs:178-187): Polygon::solve_polygon
+            // (src/polygon.rs:142-162) pair by pair, edge by edge.  For one edge
+            // the other polygon's points are independent (each reads the edge's
+            // ends as they were when its turn came, its own position and the two
+            // centres), so they run across the workgroup; the last point that
+            // hits (highest index) decides where the edge's ends go.
+            if (npoly >= 2) {
+                if (tid == 0) { s_cc->poly_last[0] = 0u; s_cc->poly_last[1] = 0u; }
+                __syncthreads();
+                uint32_t turn = 0;
+                for (uint32_t i = 0; i < npoly; i++)
+                    for (uint32_t j = i + 1; j < npoly; j++)
+                        for (uint32_t dir = 0; dir < 2; dir++) {
+                            const uint32_t si = dir ? j : i, oi = dir ? i : j;
+                            const uint4 ts = ptab[si], to = ptab[oi];
+                            const float2 sc = ctr[si], oc = ctr[oi];
+                            for (uint32_t e = 0; e < ts.y; e++, turn++) {
+                                const uint32_t ia = ts.x + e, ib = ts.x + (e + 1 == ts.y ? 0u : e + 1);
+                                const float2 pa = P[ia], pb = P[ib];
+                                uint32_t mine = 0;
+                                float2 na = pa, nb = pb;
+                                for (uint32_t k = tid; k < to.y; k += WG) {
+                                    float2 a2, b2, o2;
+                                    if (resolve_line(sc, pa, pb, P[to.x + k], oc, a2, b2, o2)) {
+                                        P[to.x + k] = o2;
+                                        na = a2; nb = b2; mine = k + 1;
+                                    }
+                                }
+                                uint32_t *slot = &s_cc->poly_last[turn & 1u];
+                                if (mine) atomicMax(slot, mine);
+                                __syncthreads();
+                                if (mine && mine == *slot) { P[ia] = na; P[ib] = nb; }
+                                // the other slot was last read before the previous barrier
+                                if (tid == 0) s_cc->poly_last[(turn & 1u) ^ 1u] = 0u;
+                                __syncthreads();
+                            }
+                        }
             }
             // ---- bodies: colliders -> bounds -> Verlet
             const uint64_t tq0 = wall_clock64();
@@ -952,26 +1054,29 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             for (uint32_t j = tid; j < N; j += WG) {
                 if (!(pins && j < Np && ((pinmask[j >> 5] >> (j & 31)) & 1u))) {
                     float2 p = P[j];
-                    float r = j >= Np ? radius(j - Np) : 0.0f;
+                    const bool circle = j >= Np && j < NpNc;
+                    float r = circle ? radius(j - Np) : 0.0f;
                     static_colliders(p, r, d.ncol, hdr, geo, pl);
                     float2 q = Q[j];
                     float ax = d.gex, ay = d.gey;
-                    if (j >= Np) {
+                    // a static polygon's points skip Polygon::update (src/polygon.rs:125-128)
+                    const bool integrate = j < NpNc || !((gstatic[(j - NpNc) >> 5] >> ((j - NpNc) & 31)) & 1u);
+                    if (circle) {
                         circle_bounds_axis(p.x, q.x, d.lox, d.hix, r);
                         circle_bounds_axis(p.y, q.y, d.loy, d.hiy, r);
-                        if (acc) { float2 a0 = acc0[j - Np]; ax = a0.x + d.gx; ay = a0.y + d.gy; }
                     } else {
                         bounds_axis(p.x, q.x, d.lox, d.hix);
                         bounds_axis(p.y, q.y, d.loy, d.hiy);
                     }
-                    verlet(p.x, p.y, q.x, q.y, ax, ay, d.delta);
+                    if (acc && j >= Np) { float2 a0 = acc0[j - Np]; ax = a0.x + d.gx; ay = a0.y + d.gy; }
+                    if (integrate) verlet(p.x, p.y, q.x, q.y, ax, ay, d.delta);
                     P[j] = p;
                     Q[j] = q;
                 }
             }
             acc = false;
             __syncthreads();
-            t_bodies += wall_clock64() - tq0;
+            if (tid == 0) s_cc->cnt.t_bodies += wall_clock64() - tq0;
         }
         ran = step + 1;
         if (d.budget && ran < steps) {
@@ -995,27 +1100,28 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         st->skin = skin;
         st->list_valid = list_valid ? 1u : 0u;
         st->err = err;
-        st->rounds += cs.rounds;
-        st->redos += cs.redos;
-        st->rebuilds += cs.rebuilds;
+        const CCShared::Counts &k = s_cc->cnt;
+        st->rounds += k.rounds;
+        st->redos += k.redos;
+        st->rebuilds += k.rebuilds;
         st->pairs = cs.pairs;
         if (Nc >= 2) st->ws_ready = 1u;
-        st->t_links += t_links;
-        st->t_build += cs.t_build;
-        st->t_sweep += t_sweep - cs.t_build;
-        st->t_bodies += t_bodies;
+        st->t_links += k.t_links;
+        st->t_build += k.t_build;
+        st->t_sweep += k.t_sweep - k.t_build;
+        st->t_bodies += k.t_bodies;
         st->since_build = cs.since_build;
-        st->n_out = cs.n_out;
-        st->n_far = cs.n_far;
-        st->max_list = cs.max_list;
+        st->n_out = k.n_out;
+        st->n_far = k.n_far;
+        st->max_list = k.max_list;
         st->s_cell = cs.s_cell;
-        st->smax = cs.smax;
+        st->smax = k.smax;
         st->smin = cs.smin;
-        st->t_count += cs.t_count;
-        st->t_lists += cs.t_lists;
-        st->t_sort += cs.t_sort;
-        st->kept += cs.kept;
-        st->unchecked += cs.unchecked;
+        st->t_count += k.t_count;
+        st->t_lists += k.t_lists;
+        st->t_sort += k.t_sort;
+        st->kept += k.kept;
+        st->unchecked += k.unchecked;
         st->last_steps = ran;
     }
     __syncthreads();
@@ -1130,11 +1236,11 @@ int variant_for_wg(int wg) {
 // ---- scene images ----------------------------------------------------------
 
 int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::string &err) {
-    if (!s.polys.empty()) { err = "soft polygons are not supported by the SCENE engine"; return BD_ERR_CAPACITY; }
     if (p.N > kSceneMaxPoints) { err = "scene too large for the SCENE engine"; return BD_ERR_CAPACITY; }
     SceneDesc &d = img.desc;
     std::memset(&d, 0, sizeof(d));
-    d.N = p.N; d.Np = p.Np; d.Nc = p.Nc; d.depth = p.depth;
+    d.N = p.N; d.Np = p.Np; d.Nc = p.Nc; d.Ng = p.Ng; d.depth = p.depth;
+    d.P = (uint32_t)s.polys.size();
     d.L = (uint32_t)p.links.size();
     d.ncol = (uint32_t)s.cols.size();
     d.nplanes = (uint32_t)(s.planes.size() / 3);
@@ -1144,13 +1250,16 @@ int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::s
     d.pos_off = (uint32_t)o; o = align16(o + 8ull * d.N);
     d.prev_off = (uint32_t)o; o = align16(o + 8ull * d.N);
     d.crad_off = (uint32_t)o; o = align16(o + 4ull * d.Nc);
-    d.acc0_off = (uint32_t)o; o = align16(o + 8ull * d.Nc);
+    d.acc0_off = (uint32_t)o; o = align16(o + 8ull * (d.Nc + d.Ng));
     d.pin_off = (uint32_t)o; o = align16(o + 4ull * ((d.Np + 31) / 32));
     d.loff_off = (uint32_t)o; o = align16(o + 4ull * (d.depth + 1));
     d.links_off = (uint32_t)o; o = align16(o + 8ull * d.L);
     d.hdr_off = (uint32_t)o; o = align16(o + 16ull * d.ncol);
     d.geo_off = (uint32_t)o; o = align16(o + 16ull * d.ncol);
     d.pl_off = (uint32_t)o; o = align16(o + 16ull * d.nplanes);
+    d.poly_off = (uint32_t)o; o = align16(o + 16ull * d.P);
+    d.ctr_off = (uint32_t)o; o = align16(o + 8ull * d.P);
+    d.gst_off = (uint32_t)o; o = align16(o + 4ull * ((d.Ng + 31) / 32));
     img.init_bytes = o;  // everything above is initialised from the host
     d.ws_off = (uint32_t)o;
     d.ws_bytes = d.Nc >= 2 ? cc_layout(d.Nc, d.T).bytes : 0u;
@@ -1162,10 +1271,27 @@ int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::s
     std::vector<V2> pos(d.N), prev(d.N);
     for (uint32_t i = 0; i < d.Np; i++) { pos[i] = s.p_pos[i]; prev[i] = s.p_prev[i]; }
     for (uint32_t i = 0; i < d.Nc; i++) { pos[d.Np + i] = s.c_pos[i]; prev[d.Np + i] = s.c_prev[i]; }
+    for (uint32_t i = 0; i < d.Ng; i++) { pos[d.Np + d.Nc + i] = s.g_pos[i]; prev[d.Np + d.Nc + i] = s.g_prev[i]; }
     put(d.pos_off, pos.data(), 8ull * d.N);
     put(d.prev_off, prev.data(), 8ull * d.N);
     put(d.crad_off, s.c_r.data(), 4ull * d.Nc);
     put(d.acc0_off, s.c_acc.data(), 8ull * d.Nc);
+    put(d.acc0_off + 8ull * d.Nc, s.g_acc.data(), 8ull * d.Ng);
+    {
+        std::vector<uint4> tab(d.P);
+        std::vector<float2> ctr(d.P);
+        std::vector<uint32_t> mask((d.Ng + 31) / 32, 0u);
+        for (uint32_t i = 0; i < d.P; i++) {
+            const Poly &pg = s.polys[i];
+            tab[i] = make_uint4(d.Np + d.Nc + pg.off, pg.n, pg.is_static ? 1u : 0u, 0u);
+            ctr[i] = make_float2(pg.center.x, pg.center.y);
+            if (pg.is_static)
+                for (uint32_t k = pg.off; k < pg.off + pg.n; k++) mask[k >> 5] |= 1u << (k & 31);
+        }
+        put(d.poly_off, tab.data(), 16ull * d.P);
+        put(d.ctr_off, ctr.data(), 8ull * d.P);
+        put(d.gst_off, mask.data(), 4ull * mask.size());
+    }
     bool pins = false;
     {
         std::vector<uint32_t> mask((d.Np + 31) / 32, 0u);
@@ -1215,6 +1341,33 @@ int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::s
     return BD_OK;
 }
 
+void scene_chunk_to_host(const SceneDesc &d, const uint8_t *chunk, HostScene &h) {
+    const float2 *pos = (const float2 *)(chunk + d.pos_off);
+    const float2 *prev = (const float2 *)(chunk + d.prev_off);
+    const float2 *ctr = (const float2 *)(chunk + d.ctr_off);
+    const uint32_t NpNc = d.Np + d.Nc;
+    h.p_pos.resize(d.Np); h.p_prev.resize(d.Np);
+    h.c_pos.resize(d.Nc); h.c_prev.resize(d.Nc);
+    h.g_pos.resize(d.Ng); h.g_prev.resize(d.Ng);
+    for (uint32_t i = 0; i < d.N; i++) {
+        V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
+        if (i < d.Np) { h.p_pos[i] = p; h.p_prev[i] = q; }
+        else if (i < NpNc) { h.c_pos[i - d.Np] = p; h.c_prev[i - d.Np] = q; }
+        else { h.g_pos[i - NpNc] = p; h.g_prev[i - NpNc] = q; }
+    }
+    // the acc the bodies held when the chunk was built (the kernel reads it
+    // on the first substep and never writes it)
+    const float2 *acc0 = (const float2 *)(chunk + d.acc0_off);
+    h.c_acc.resize(d.Nc);
+    h.g_acc.resize(d.Ng);
+    for (uint32_t i = 0; i < d.Nc; i++) h.c_acc[i] = V2{acc0[i].x, acc0[i].y};
+    for (uint32_t i = 0; i < d.Ng; i++) h.g_acc[i] = V2{acc0[d.Nc + i].x, acc0[d.Nc + i].y};
+    h.polys.assign(d.P, Poly{});
+    for (uint32_t i = 0; i < d.P; i++) h.polys[i].center = V2{ctr[i].x, ctr[i].y};
+    h.gravity = V2{d.gx, d.gy};
+    h.sub_steps = d.sub_steps;
+}
+
 void set_scene_params(SceneDesc &d, const StepParams &sp, uint32_t steps, bool acc_pending) {
     d.steps = steps;
     d.sub_steps = sp.sub_steps;
@@ -1260,7 +1413,7 @@ public:
         const size_t with_links = 16ull * desc_.N + 8ull * desc_.L;
         lds_bytes_ = with_links <= scene_max_lds() ? with_links : std::min<size_t>(16ull * desc_.N, scene_max_lds());
         BD_HIP(hipStreamSynchronize(stream_));
-        N_ = p.N; Np_ = p.Np;
+        N_ = p.N; Np_ = p.Np; Nc_ = p.Nc; P_ = desc_.P;
         return BD_OK;
     }
 
@@ -1280,24 +1433,38 @@ public:
         return BD_OK;
     }
 
-    int download(HostScene &s, std::string &err) override {
-        if (!host_shape_matches(s, Np_, N_ - Np_, 0, err)) return BD_ERR_STATE;
-        std::vector<float2> pos(N_), prev(N_);
-        SceneState stt;
+    // pos[N], prev[N], the scene's SceneState, then the polygon centres
+    static constexpr size_t kStateWords = (sizeof(SceneState) + sizeof(float2) - 1) / sizeof(float2);
+    size_t state_count() const override { return 2ull * N_ + kStateWords + P_; }
+    hipStream_t stream() const override { return stream_; }
+
+    int copy_state(void *dst, hipMemcpyKind kind, hipStream_t st, std::string &err) override {
+        float2 *d = (float2 *)dst;
         if (N_) {
-            BD_HIP(hipMemcpyAsync(pos.data(), chunk_ + desc_.pos_off, 8ull * N_, hipMemcpyDeviceToHost, stream_));
-            BD_HIP(hipMemcpyAsync(prev.data(), chunk_ + desc_.prev_off, 8ull * N_, hipMemcpyDeviceToHost, stream_));
+            BD_HIP(hipMemcpyAsync(d, chunk_ + desc_.pos_off, 8ull * N_, kind, st));
+            BD_HIP(hipMemcpyAsync(d + N_, chunk_ + desc_.prev_off, 8ull * N_, kind, st));
         }
-        BD_HIP(hipMemcpyAsync(&stt, state_ptr(), sizeof(stt), hipMemcpyDeviceToHost, stream_));
-        BD_HIP(hipStreamSynchronize(stream_));
-        for (uint32_t i = 0; i < Np_; i++) {
-            s.p_pos[i] = V2{pos[i].x, pos[i].y};
-            s.p_prev[i] = V2{prev[i].x, prev[i].y};
+        BD_HIP(hipMemcpyAsync(d + 2 * N_, state_ptr(), sizeof(SceneState), kind, st));
+        if (P_) BD_HIP(hipMemcpyAsync(d + 2 * N_ + kStateWords, chunk_ + desc_.ctr_off, 8ull * P_, kind, st));
+        return BD_OK;
+    }
+
+    int unpack(const float2 *src, HostScene &s, std::string &err) override {
+        if (!host_shape_matches(s, Np_, Nc_, N_ - Np_ - Nc_, err) || s.polys.size() != P_) return BD_ERR_STATE;
+        const float2 *pos = src, *prev = src + N_, *ctr = src + 2 * N_ + kStateWords;
+        SceneState stt;
+        std::memcpy(&stt, src + 2 * N_, sizeof(stt));
+        const uint32_t NpNc = Np_ + Nc_;
+        for (uint32_t i = 0; i < N_; i++) {
+            V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
+            if (i < Np_) { s.p_pos[i] = p; s.p_prev[i] = q; }
+            else if (i < NpNc) { s.c_pos[i - Np_] = p; s.c_prev[i - Np_] = q; }
+            else { s.g_pos[i - NpNc] = p; s.g_prev[i - NpNc] = q; }
         }
-        for (uint32_t i = Np_; i < N_; i++) {
-            s.c_pos[i - Np_] = V2{pos[i].x, pos[i].y};
-            s.c_prev[i - Np_] = V2{prev[i].x, prev[i].y};
-        }
+        // a static polygon keeps the centre of its last Polygon::solve_links
+        // (the others' centres follow from prev on the host)
+        for (uint32_t i = 0; i < P_; i++)
+            if (s.polys[i].is_static) s.polys[i].center = V2{ctr[i].x, ctr[i].y};
         if (stt.err) {
             err = "circle pair schedule did not converge (kind " + std::to_string(stt.err) + ", skin " +
                   std::to_string(stt.skin) + ", rounds " + std::to_string(stt.rounds) + ", redos " +
@@ -1322,11 +1489,12 @@ private:
     bool acc_pending_ = false;
     int variant_ = 0;
     size_t lds_bytes_ = 0;
-    uint32_t N_ = 0, Np_ = 0;
+    uint32_t N_ = 0, Np_ = 0, Nc_ = 0, P_ = 0;
 };
 
 bool scene_engine_fits(const HostScene &s, const Plan &p) {
-    return s.polys.empty() && p.N <= 16384 && p.N > 0;
+    (void)s;
+    return p.N <= 16384 && p.N > 0;
 }
 
 Engine *make_scene_engine(hipStream_t s) { return new SceneEngine(s); }

@@ -232,13 +232,21 @@ public:
         return BD_OK;
     }
 
-    int download(HostScene &s, std::string &err) override {
+    size_t state_count() const override { return 2ull * N_; }
+    hipStream_t stream() const override { return stream_; }
+
+    int copy_state(void *dst, hipMemcpyKind kind, hipStream_t st, std::string &err) override {
+        float2 *d = (float2 *)dst;
+        if (N_) {
+            BD_HIP(hipMemcpyAsync(d, pos_[cur_].p, N_ * sizeof(float2), kind, st));
+            BD_HIP(hipMemcpyAsync(d + N_, prev_.p, N_ * sizeof(float2), kind, st));
+        }
+        return BD_OK;
+    }
+
+    int unpack(const float2 *src, HostScene &s, std::string &err) override {
         if (!host_shape_matches(s, N_, 0, 0, err)) return BD_ERR_STATE;
-        if (!N_) return BD_OK;
-        std::vector<float2> pos(N_), prev(N_);
-        BD_HIP(hipMemcpyAsync(pos.data(), pos_[cur_].p, N_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));
-        BD_HIP(hipMemcpyAsync(prev.data(), prev_.p, N_ * sizeof(float2), hipMemcpyDeviceToHost, stream_));
-        BD_HIP(hipStreamSynchronize(stream_));
+        const float2 *pos = src, *prev = src + N_;
         for (uint32_t i = 0; i < N_; i++) {
             s.p_pos[perm_[i]] = V2{pos[i].x, pos[i].y};
             s.p_prev[perm_[i]] = V2{prev[i].x, prev[i].y};

@@ -4,13 +4,15 @@
 // Every scene lives in one contiguous chunk of device memory (so scenes can be
 // admitted, evicted and repacked independently):
 //
-//   pos[N] float2 | prev[N] float2 | crad[Nc] f32 | acc0[Nc] float2 |
+//   pos[N] float2 | prev[N] float2 | crad[Nc] f32 | acc0[Nc+Ng] float2 |
 //   pinmask[ceil(Np/32)] u32 | layer_off[depth+1] u32 | links[L] LinkS |
 //   collider hdr[ncol] int4 | geo[ncol] float4 | planes[nplanes] float4 |
+//   polygons[P] uint4 | centres[P] float2 | static mask[ceil(Ng/32)] u32 |
 //   circle-pair workspace (if Nc >= 2)
 //
-// Scene-local point order: particles [0, Np), circles [Np, N).  Soft
-// polygons are not held by this engine.
+// Scene-local point order: particles [0, Np), circles [Np, Np+Nc), soft
+// polygon points [Np+Nc, N), polygon by polygon.  A polygon record is
+// (first point, point count, is_static, 0).
 #pragma once
 
 #include <cstdint>
@@ -49,6 +51,9 @@ struct SceneDesc {
     uint32_t pad0;
     uint64_t ws_base;            // device address of the circle-pair workspace when it is not
                                  // in the chunk (0: base + ws_off)
+    uint32_t Ng, P;              // soft polygon points and polygons
+    uint32_t poly_off, ctr_off;  // polygon records (uint4) and their centres (float2)
+    uint32_t gst_off, pad1;      // static-polygon point mask (one bit per polygon point)
 };
 static_assert(sizeof(SceneDesc) <= 256, "a scene descriptor fits its 256-byte slot");
 

@@ -9,7 +9,7 @@
 //               updates-ran flag, format version
 //   [384, 512)  SceneState: the circle-pair schedule's state between updates
 //   [512, ...)  the chunk's initialised part: pos, prev, radii, initial acc,
-//               pin mask, link layers, links, static colliders
+//               pin mask, link layers, links, static colliders, polygons
 //
 // An update is two launches on the caller's stream: k_image_begin (one lane)
 // writes the per-call fields of the header on the device, then the SCENE
@@ -39,13 +39,13 @@ using namespace bd;
 namespace {
 
 constexpr uint64_t kImgWords = 256, kImgState = 384, kImgChunk = 512;
-constexpr uint32_t kImgVersion = 0x42440001u;  // "BD", layout 1
+constexpr uint32_t kImgVersion = 0x42440002u;  // "BD", layout 2
 constexpr uint32_t kErrWorkspace = 16u;        // SceneState::err: the workspace was too small
 
 static_assert(sizeof(SceneState) <= kImgChunk - kImgState, "scene state fits the header");
 
 struct ImageWords {
-    uint32_t order0, queue, ran, version;
+    uint32_t order0, queue, ran, version;  // ran: updates run so far
 };
 
 int fail(int code, const std::string &msg) {
@@ -72,7 +72,7 @@ int image_of(const bd_solver *s, SceneImage &img, std::string &err) {
     rc = build_plan(h, p, err);
     if (rc) return rc;
     if (!scene_engine_fits(h, p)) {
-        err = "scene images hold scenes without soft polygons of 1 to 16384 bodies";
+        err = "scene images hold scenes of 1 to 16384 bodies";
         return BD_ERR_CAPACITY;
     }
     rc = build_scene_image(h, p, img, err);
@@ -111,7 +111,7 @@ __global__ void k_image_begin(uint8_t *img, uint8_t *ws, uint64_t ws_cap, float 
         return;
     }
     d->steps = steps;
-    if (steps) w->ran = 1u;
+    w->ran += steps;
 }
 
 // Shared by the workspace and the allocation-free update: validates the call
@@ -252,19 +252,11 @@ int32_t bd_scene_image_read(bd_solver *s, const void *host_image, uint64_t bytes
     std::memcpy(&w, b + kImgWords, sizeof(w));
     std::memcpy(&st, b + kImgState, sizeof(st));
     if (w.version != kImgVersion) return fail(BD_ERR_ARG, "not a scene image");
-    if (bytes < kImgChunk + (uint64_t)d.prev_off + 8ull * d.N) return fail(BD_ERR_ARG, "image truncated");
-    const float2 *pos = (const float2 *)(b + kImgChunk + d.pos_off);
-    const float2 *prev = (const float2 *)(b + kImgChunk + d.prev_off);
+    if (bytes < kImgChunk + (uint64_t)d.ws_off) return fail(BD_ERR_ARG, "image truncated");
     HostScene h;
-    h.p_pos.resize(d.Np); h.p_prev.resize(d.Np);
-    h.c_pos.resize(d.Nc); h.c_prev.resize(d.Nc);
-    for (uint32_t i = 0; i < d.N; i++) {
-        V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
-        if (i < d.Np) { h.p_pos[i] = p; h.p_prev[i] = q; }
-        else { h.c_pos[i - d.Np] = p; h.c_prev[i - d.Np] = q; }
-    }
+    scene_chunk_to_host(d, b + kImgChunk, h);
     std::string err;
-    int rc = solver_import_state(s, h, w.ran ? 1u : 0u, err);
+    int rc = solver_import_state(s, h, w.ran, err);
     if (rc) return fail(rc, err);
     if (st.err == kErrWorkspace) return fail(BD_ERR_CAPACITY, "an update was given a workspace smaller than the image needs");
     if (st.err) return fail(BD_ERR_COLLISION, "circle pair schedule failed for this scene");

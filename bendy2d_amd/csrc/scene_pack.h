@@ -25,6 +25,14 @@ struct SceneImage {
 };
 
 int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::string &err);
+
+// The bodies of one scene from a host copy of the first init_bytes of its
+// chunk: particles, circles and polygon points by index, the acc of circles
+// and polygon points when the chunk was built, each polygon's last centre
+// (h.polys gets one record per polygon, holding only that centre), and the
+// gravity and substeps the scene runs with (for the acc that static polygon
+// points accumulate).
+void scene_chunk_to_host(const SceneDesc &d, const uint8_t *chunk, HostScene &h);
 void set_scene_params(SceneDesc &d, const StepParams &sp, uint32_t steps, bool acc_pending);
 
 // Kernel variants by workgroup size.  A scene of N points keeps pos and prev

@@ -403,7 +403,7 @@ int64_t bd_server_submit_create(bd_server *S, const bd_solver *scene, uint32_t s
     Plan p;
     rc = build_plan(h, p, err);
     if (rc) return fail(rc, err);
-    if (!h.polys.empty() || p.N > 16384) return fail(BD_ERR_CAPACITY, "served scenes hold no soft polygons and at most 16384 bodies");
+    if (p.N > 16384) return fail(BD_ERR_CAPACITY, "served scenes hold at most 16384 bodies");
     SceneImage img;
     rc = build_scene_image(h, p, img, err);
     if (rc) return fail(rc, err);
@@ -416,6 +416,9 @@ int64_t bd_server_submit_create(bd_server *S, const bd_solver *scene, uint32_t s
     sc->saved.skin = img.skin0;
     sc->params = make_params(h, dt, img.acc_pending);
     sc->acc_pending = img.acc_pending;
+    // gravity and substeps in the stored descriptor too: reading the scene
+    // back derives the acc of static polygon points from them
+    set_scene_params(sc->desc, sc->params, 0, sc->acc_pending);
     if (sc->total_bytes > S->arena.size) return fail(BD_ERR_BUDGET, "scene larger than the memory budget");
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->fatal.empty()) return fail(BD_ERR_DEVICE, S->fatal);
@@ -482,16 +485,8 @@ int32_t bd_server_read_scene(bd_server *S, int64_t scene_id, bd_solver *out) {
     } else {
         buf = sc->host;
     }
-    const float2 *pos = (const float2 *)(buf.data() + d.pos_off);
-    const float2 *prev = (const float2 *)(buf.data() + d.prev_off);
     HostScene h;
-    h.p_pos.resize(d.Np); h.p_prev.resize(d.Np);
-    h.c_pos.resize(d.Nc); h.c_prev.resize(d.Nc);
-    for (uint32_t i = 0; i < d.N; i++) {
-        V2 p{pos[i].x, pos[i].y}, q{prev[i].x, prev[i].y};
-        if (i < d.Np) { h.p_pos[i] = p; h.p_prev[i] = q; }
-        else { h.c_pos[i - d.Np] = p; h.c_prev[i - d.Np] = q; }
-    }
+    scene_chunk_to_host(d, buf.data(), h);
     std::string err;
     int rc = solver_import_state(out, h, sc->updates_run, err);
     if (rc) return fail(rc, err);

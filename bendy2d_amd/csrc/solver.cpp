@@ -63,6 +63,29 @@ struct bd_solver {
     hipStream_t stream = nullptr;
     int device = -1;
     Stats stats;
+    // Readback (bd_ext_set_readback): after each update, a device-side copy of
+    // the state into one of two staging buffers on the solver's stream, then a
+    // copy to pinned host memory on a second stream, which overlaps the next
+    // update's kernels.  `slot` holds the latest update's copy when `valid`.
+    struct Readback {
+        bool on = false, valid = false;
+        int slot = -1;
+        DevBuf<float2> dev[2];
+        float2 *host[2] = {nullptr, nullptr};
+        size_t host_cap[2] = {0, 0};
+        hipEvent_t snap[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+        bool used[2] = {false, false};
+        hipStream_t copy = nullptr;
+    } rb;
+    ~bd_solver() {
+        for (int k = 0; k < 2; k++) {
+            if (rb.done[k]) (void)hipEventSynchronize(rb.done[k]);
+            if (rb.host[k]) (void)hipHostFree(rb.host[k]);
+            if (rb.snap[k]) (void)hipEventDestroy(rb.snap[k]);
+            if (rb.done[k]) (void)hipEventDestroy(rb.done[k]);
+        }
+        if (rb.copy) (void)hipStreamDestroy(rb.copy);
+    }
 };
 
 static int fail(int code, const std::string &msg) {
@@ -70,25 +93,36 @@ static int fail(int code, const std::string &msg) {
     return code;
 }
 
-// Copy device state back into the host scene if the device is newer.
+// Polygon::update recomputes the centre from the positions it is about to
+// integrate (src/polygon.rs:130), which are prev afterwards.
+static void centre_from_prev(HostScene &h, size_t p) {
+    Poly &pg = h.polys[p];
+    float cx = 0.0f, cy = 0.0f;
+    for (uint32_t i = 0; i < pg.n; i++) {
+        cx = cx + h.g_prev[pg.off + i].x;
+        cy = cy + h.g_prev[pg.off + i].y;
+    }
+    pg.center = V2{cx / (float)pg.n, cy / (float)pg.n};
+}
+
+// Copy device state back into the host scene if the device is newer: from
+// the readback copy of the latest update when there is one, else directly.
 static int ensure_host(const bd_solver *cs) {
     bd_solver *s = const_cast<bd_solver *>(cs);
     if (!s->dev_newer || !s->eng) return BD_OK;
     std::string err;
-    int rc = s->eng->download(s->host, err);
-    if (rc) return fail(rc, err);
-    // Polygon::update recomputes the centre from the positions it is about to
-    // integrate (src/polygon.rs:130), which are prev afterwards.
-    for (size_t p = 0; p < s->host.polys.size(); p++) {
-        Poly &pg = s->host.polys[p];
-        if (pg.is_static || p >= s->center_from_prev.size() || !s->center_from_prev[p]) continue;
-        float cx = 0.0f, cy = 0.0f;
-        for (uint32_t i = 0; i < pg.n; i++) {
-            cx = cx + s->host.g_prev[pg.off + i].x;
-            cy = cy + s->host.g_prev[pg.off + i].y;
-        }
-        pg.center = V2{cx / (float)pg.n, cy / (float)pg.n};
+    int rc;
+    if (s->rb.valid) {
+        const int k = s->rb.slot;
+        if (hipEventSynchronize(s->rb.done[k]) != hipSuccess) return fail(BD_ERR_DEVICE, "readback copy failed");
+        rc = s->eng->unpack(s->rb.host[k], s->host, err);
+    } else {
+        rc = s->eng->download(s->host, err);
     }
+    if (rc) return fail(rc, err);
+    for (size_t p = 0; p < s->host.polys.size(); p++)
+        if (!s->host.polys[p].is_static && p < s->center_from_prev.size() && s->center_from_prev[p])
+            centre_from_prev(s->host, p);
     s->dev_newer = false;
     return BD_OK;
 }
@@ -123,18 +157,43 @@ int solver_export(const bd_solver *s, HostScene &out, std::string &err) {
 }
 
 // Overwrite the body state of `s` with that of `src` (same scene shape), as
-// left by `updates` updates run elsewhere.
+// left by `updates` updates run elsewhere with src.gravity and src.sub_steps
+// from the acc in src.c_acc / src.g_acc.  src.polys holds each polygon's
+// last centre.
 int solver_import_state(bd_solver *s, const HostScene &src, uint64_t updates, std::string &err) {
     int rc = ensure_host(s);
     if (rc) { err = bd_last_error_str(); return rc; }
     HostScene &h = s->host;
-    if (h.p_pos.size() != src.p_pos.size() || h.c_pos.size() != src.c_pos.size() || !h.polys.empty()) {
+    if (h.p_pos.size() != src.p_pos.size() || h.c_pos.size() != src.c_pos.size() ||
+        h.g_pos.size() != src.g_pos.size() || h.polys.size() != src.polys.size()) {
         err = "scene shape differs from the batch scene";
         return BD_ERR_ARG;
     }
     h.p_pos = src.p_pos; h.p_prev = src.p_prev;
     h.c_pos = src.c_pos; h.c_prev = src.c_prev;
-    if (updates) for (V2 &a : h.c_acc) a = V2{0.0f, 0.0f};
+    h.g_pos = src.g_pos; h.g_prev = src.g_prev;
+    if (updates) {
+        // what settle_acc does after a local update: acc spent, except static
+        // polygon points, which gain gravity every substep
+        const uint64_t substeps = updates * src.sub_steps;
+        for (V2 &a : h.c_acc) a = V2{0.0f, 0.0f};
+        for (size_t p = 0; p < h.polys.size(); p++) {
+            const Poly &pg = h.polys[p];
+            for (uint32_t i = 0; i < pg.n; i++) {
+                V2 &a = h.g_acc[pg.off + i];
+                if (pg.is_static) {
+                    a = src.g_acc[pg.off + i];
+                    for (uint64_t k = 0; k < substeps; k++) a = V2{a.x + src.gravity.x, a.y + src.gravity.y};
+                } else {
+                    a = V2{0.0f, 0.0f};
+                }
+            }
+            if (pg.is_static) h.polys[p].center = src.polys[p].center;
+            else centre_from_prev(h, p);
+            if (p < s->center_from_prev.size()) s->center_from_prev[p] = pg.is_static ? 0 : 1;
+        }
+        s->acc_pending = false;
+    }
     s->plan_valid = false;  // device copy (if any) is stale now
     s->dev_newer = false;
     return BD_OK;
@@ -576,6 +635,45 @@ static int ensure_device(bd_solver *s) {
 
 static void settle_acc(bd_solver *s, uint64_t substeps);
 
+// Queue the readback of the state the last update left (see bd_solver::rb).
+static int readback_begin(bd_solver *s) {
+    bd_solver::Readback &rb = s->rb;
+    const size_t n = s->eng->state_count();
+    const int k = rb.slot < 0 ? 0 : (rb.slot ^ 1);
+    auto hip_ok = [](hipError_t e) { return e == hipSuccess; };
+    if (!rb.copy) {
+        if (!hip_ok(hipStreamCreateWithFlags(&rb.copy, hipStreamNonBlocking))) return fail(BD_ERR_DEVICE, "readback stream");
+        for (int j = 0; j < 2; j++)
+            if (!hip_ok(hipEventCreateWithFlags(&rb.snap[j], hipEventDisableTiming)) ||
+                !hip_ok(hipEventCreateWithFlags(&rb.done[j], hipEventDisableTiming)))
+                return fail(BD_ERR_DEVICE, "readback events");
+    }
+    // slot k's previous host copy must be finished before its device
+    // staging is overwritten (on the solver's stream) or its host buffer grows
+    if (rb.used[k] && !hip_ok(hipStreamWaitEvent(s->stream, rb.done[k], 0))) return fail(BD_ERR_DEVICE, "readback wait");
+    if (rb.dev[k].alloc(std::max<size_t>(n, 1)) != hipSuccess) return fail(BD_ERR_NOMEM, "readback staging");
+    if (rb.host_cap[k] < n) {
+        if (rb.used[k] && !hip_ok(hipEventSynchronize(rb.done[k]))) return fail(BD_ERR_DEVICE, "readback wait");
+        if (rb.host[k]) (void)hipHostFree(rb.host[k]);
+        rb.host[k] = nullptr;
+        rb.host_cap[k] = 0;
+        if (!hip_ok(hipHostMalloc((void **)&rb.host[k], n * sizeof(float2), hipHostMallocDefault)))
+            return fail(BD_ERR_NOMEM, "readback pinned buffer");
+        rb.host_cap[k] = n;
+    }
+    std::string err;
+    int rc = s->eng->copy_state(rb.dev[k].p, hipMemcpyDeviceToDevice, s->stream, err);
+    if (rc) return fail(rc, err);
+    if (!hip_ok(hipEventRecord(rb.snap[k], s->stream)) || !hip_ok(hipStreamWaitEvent(rb.copy, rb.snap[k], 0)) ||
+        (n && !hip_ok(hipMemcpyAsync(rb.host[k], rb.dev[k].p, n * sizeof(float2), hipMemcpyDeviceToHost, rb.copy))) ||
+        !hip_ok(hipEventRecord(rb.done[k], rb.copy)))
+        return fail(BD_ERR_DEVICE, "readback copy");
+    rb.used[k] = true;
+    rb.slot = k;
+    rb.valid = true;
+    return BD_OK;
+}
+
 int32_t bd_update(bd_solver *s, float dt) {
     if (!s) return BD_ERR_ARG;
     std::string err;
@@ -613,6 +711,7 @@ int32_t bd_update(bd_solver *s, float dt) {
         s->engine_used = want;
     }
     StepParams sp = make_params(s->host, dt, s->acc_pending);
+    s->rb.valid = false;  // a readback copy, if any, is of the state before this update
     const uint64_t substeps_before = s->stats.substeps;
     int rc = s->eng->update(sp, s->stats, err);
     if (rc) {
@@ -625,6 +724,14 @@ int32_t bd_update(bd_solver *s, float dt) {
     s->stats.updates++;
     settle_acc(s, s->host.sub_steps);
     s->dev_newer = true;
+    if (s->rb.on) return readback_begin(s);
+    return BD_OK;
+}
+
+int32_t bd_ext_set_readback(bd_solver *s, int32_t on) {
+    if (!s) return BD_ERR_ARG;
+    s->rb.on = on != 0;
+    if (!s->rb.on) s->rb.valid = false;
     return BD_OK;
 }
 

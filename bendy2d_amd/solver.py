@@ -284,6 +284,11 @@ class Solver:
     def sync(self):
         check(lib().bd_ext_sync(self._h), "sync")
 
+    def set_readback(self, on=True):
+        """Copy every update's state to pinned host memory as it finishes
+        (bd_ext_set_readback), so getters after an update only wait for it."""
+        check(lib().bd_ext_set_readback(self._h, int(bool(on))), "set_readback")
+
     def stats(self):
         s = Stats()
         check(lib().bd_ext_get_stats(self._h, ctypes.byref(s)), "stats")

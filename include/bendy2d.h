@@ -246,6 +246,15 @@ int32_t bd_ext_set_tiling(bd_solver *s, uint32_t tile_points, uint32_t workgroup
  * BD_ERR_*; passes receives the passes per sweep. */
 int64_t bd_ext_tiled_selfcheck(const bd_solver *s, uint32_t tile_points, uint32_t region_cap, uint32_t *passes);
 
+/* Readback of every update (no counterpart in the reference, whose getters
+ * lend live state, src/solver.rs:86-104).  Off (default): a getter after an
+ * update copies the state back then, synchronously.  On: each bd_update also
+ * queues a copy of the new state into pinned host memory, snapshotted on the
+ * device and copied on a second stream so that it overlaps the next update's
+ * kernels; a getter then only waits for that copy.  Values are the same
+ * either way. */
+int32_t bd_ext_set_readback(bd_solver *s, int32_t on);
+
 /* Wait for all queued device work of this solver. */
 int32_t bd_ext_sync(bd_solver *s);
 
@@ -278,8 +287,8 @@ int64_t bd_plan_layers(const uint32_t *a, const uint32_t *b, uint64_t n_links, u
 /* Batched form: many independent scenes in one persistent launch      */
 /* ------------------------------------------------------------------ */
 /* No counterpart in the reference (it has one Solver per world).  A batch
- * holds copies of independent scenes (no soft polygons, at most 16384 bodies
- * each) resident in device memory; one persistent kernel launch per size
+ * holds copies of independent scenes (particles, circles, soft polygons and
+ * static colliders; at most 16384 bodies each) resident in device memory; one persistent kernel launch per size
  * class advances all of them, one workgroup per scene taken from a work
  * queue, largest first.  Results per scene are bit-identical to bd_update on
  * the same scene. */
@@ -331,7 +340,7 @@ int32_t bd_batch_last_timing(const bd_batch *b, float *ms, uint32_t *launches);
 int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64_t *rebuilds,
                           uint64_t *err_scenes);
 /* Circle-pair schedule state of one scene: base skin, error kind (0 none,
- * 1 candidate capacity, 2 no valid sweep in 13 attempts, 3 stalled
+ * 1 candidate capacity, 2 no valid sweep in 21 attempts, 3 stalled
  * schedule), rounds, redos, rebuilds and candidate pairs. */
 int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *err, uint32_t *rounds,
                               uint32_t *redos, uint32_t *rebuilds, uint32_t *pairs);
@@ -415,7 +424,7 @@ int32_t bd_server_get_stats(bd_server *srv, bd_server_stats *out);
 /* ------------------------------------------------------------------ */
 /* No counterpart in the reference.  For callers that manage device memory
  * and streams themselves (code under HIP graph capture, serving loops): a
- * scene image is one scene (no soft polygons, 1 to 16384 bodies) in the
+ * scene image is one scene (1 to 16384 bodies of any kind) in the
  * SCENE engine's device layout behind a 512-byte header.  The caller copies
  * the host image to device memory it owns (256-byte aligned) and advances it
  * with bd_scene_image_update on its own stream: two kernel launches, no

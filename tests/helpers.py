@@ -101,3 +101,46 @@ def first_diff(x, y):
         return None
     i = int(d[0])
     return i, float(np.asarray(x)[i]), float(np.asarray(y)[i]), len(d)
+
+
+def state_acc(s):
+    """state(s) plus the acc of every circle and polygon point (static polygon
+    points keep adding gravity; everything else is spent by the update)."""
+    parts = [state(s), s.get_circles()[:, 4:6].astype(np.float64).ravel()]
+    for i in range(s.get_polygons_len()):
+        parts.append(np.asarray(s.get_polygon(i)["particles"], np.float64)[:, 4:6].ravel())
+    return np.concatenate(parts)
+
+
+def polygon_scene(seed, n_polys=6, circles=True, cloth=True, sub_steps=2, link_iters=2):
+    """Soft polygons (3 to 19 points, some with an initial acc) falling onto a
+    static polygon floor and onto each other, with a static polygon collider,
+    a static circle and, optionally, free circles and a pinned cloth."""
+    from bendy2d_amd.solver import polygon_circle
+
+    rng = np.random.default_rng(seed)
+    s = Solver()
+    s.gravity = (0.0, 60.0)
+    s.bounds = ((0.0, 0.0), (100.0, 100.0))
+    for k in range(n_polys):
+        n = int(rng.integers(3, 20))
+        pts, links, c = polygon_circle(float(rng.uniform(3.0, 7.0)),
+                                       (float(rng.uniform(15.0, 85.0)), float(rng.uniform(10.0, 70.0))), n, False)
+        if k % 3 == 1:
+            pts[:, 4:6] = rng.normal(0.0, 30.0, (n, 2)).astype(np.float32)
+        s.add_polygon(pts, links, False, c, 1.0)
+    s.add_polygon_new([(5.0, 88.0), (95.0, 88.0), (95.0, 96.0), (5.0, 96.0)], True)
+    s.add_polygon_circle(4.0, (50.0, 80.0), 9, True)
+    if circles:
+        for k in range(24):
+            s.add_circle((float(rng.uniform(10.0, 90.0)), float(rng.uniform(5.0, 60.0))),
+                         float(rng.uniform(0.8, 1.6)))
+        s.add_circle_link(0, 1, 3.0)
+    if cloth:
+        from bendy2d_amd import workloads
+        workloads.add_grid(s, 6, 5, x0=70.0, y0=20.0, pin_top=True)
+    s.add_static_circle((30.0, 75.0), 5.0)
+    s.add_static_polygon([(60.0, 60.0), (70.0, 60.0), (65.0, 68.0)])
+    s.set_sub_steps(sub_steps)
+    s.set_link_iterations(link_iters)
+    return s

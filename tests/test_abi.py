@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from helpers import ROOT, bits_equal, gpu_available, state, to_oracle
+from helpers import ROOT, bits_equal, gpu_available, state, state_acc, to_oracle
 from bendy2d_amd import Solver, lib, polygon_circle, polygon_new, workloads
 from bendy2d_amd import _lib
 from oracle import OracleSolver
@@ -171,11 +171,19 @@ def test_scene_image_host_roundtrip():
     bad[256 + 12:256 + 16] = 0  # format version word
     with pytest.raises(_lib.BendyError):
         SceneImage.read(c, bad)
-    # soft polygons are not held by scene images
-    p = Solver()
-    p.add_polygon_circle(3.0, (5.0, 5.0), 6)
+    # a scene with soft polygons, circles and particles reads back as it was
+    p = _mixed_scene()
+    before = state_acc(p)
+    img = SceneImage(p)
+    assert img.info.n_bodies == p.stats()["n_points"] and img.info.n_circles == len(p.get_circles())
+    q = p.clone()
+    SceneImage.read(q, img.host)
+    assert bits_equal(state_acc(q), before)
+    # more than an image holds is refused
+    big = Solver()
+    workloads.add_grid(big, 130, 130, as_circles=True, radius=0.3)
     with pytest.raises(_lib.BendyError) as e:
-        SceneImage(p)
+        SceneImage(big)
     assert e.value.code == _lib.BD_ERR_CAPACITY
 
 

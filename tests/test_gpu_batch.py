@@ -4,7 +4,7 @@ modes, graph replay, updates per launch)."""
 import numpy as np
 import pytest
 
-from helpers import bits_equal, first_diff, state, to_oracle
+from helpers import bits_equal, first_diff, polygon_scene, state, state_acc, to_oracle
 from bendy2d_amd import Batch, Solver, workloads
 
 pytestmark = pytest.mark.gpu
@@ -72,6 +72,32 @@ def test_batch_matches_oracle(cfg):
         b.read_scene(i, s)
         got = state(s)
         assert bits_equal(got, want[i]), f"scene {i}: {first_diff(got, want[i])}"
+
+
+def test_batch_polygon_scenes():
+    # scenes with soft polygons (and circles, cloth, static colliders) beside
+    # plain ones, in every size class; acc of static polygon points included
+    scenes = [polygon_scene(500 + i, n_polys=3 + i % 5, circles=i % 2 == 0, cloth=i % 3 != 2) for i in range(6)]
+    scenes += make_scenes(3, max_side=12)
+    want = []
+    for s in scenes:
+        o = to_oracle(s, 32)
+        for _ in range(20):
+            assert o.update(DT) == 0
+        want.append(state_acc(o))
+    for cfg in (CONFIGS[1], CONFIGS[3]):
+        cl = [x.clone() for x in scenes]
+        b = Batch()
+        for x in cl:
+            b.add_scene(x)
+        b.config = cfg
+        b.update(DT, 7)
+        b.update(DT, 13)
+        b.sync()
+        for i, x in enumerate(cl):
+            b.read_scene(i, x)
+            got = state_acc(x)
+            assert bits_equal(got, want[i]), f"scene {i}: {first_diff(got, want[i])}"
 
 
 def test_batch_autotune_keeps_state():

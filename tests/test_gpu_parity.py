@@ -7,7 +7,7 @@ position, previous position and polygon centre.
 import numpy as np
 import pytest
 
-from helpers import bits_equal, first_diff, state, to_oracle
+from helpers import bits_equal, first_diff, polygon_scene, state, state_acc, to_oracle
 from bendy2d_amd import BD_ENGINE_GLOBAL, BD_ENGINE_SCENE, BD_ENGINE_TILED, BD_ORDER_PARTICLE, Solver, workloads
 
 ENGINES = [BD_ENGINE_GLOBAL, BD_ENGINE_SCENE]
@@ -153,7 +153,7 @@ def test_circles_links_acc(engine):
     run_pair(s, 60, 1.0 / 60.0, check_every=20)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_polygons(engine):
     s = Solver()
     s.set_engine(engine)
@@ -164,7 +164,7 @@ def test_polygons(engine):
     run_pair(s, 120, 1.0 / 60.0, check_every=20)
 
 
-@pytest.mark.parametrize("engine", [BD_ENGINE_GLOBAL])
+@pytest.mark.parametrize("engine", ENGINES)
 def test_mixed_scene(engine):
     s = Solver()
     s.set_engine(engine)
@@ -177,6 +177,37 @@ def test_mixed_scene(engine):
     s.set_sub_steps(3)
     s.set_link_iterations(2)
     run_pair(s, 60, 1.0 / 60.0, check_every=20)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("seed", [11, 12])
+def test_polygon_pile(engine, seed):
+    # soft polygons landing on each other and on static polygons, among
+    # circles and a cloth; some polygon points start with an acc
+    s = polygon_scene(seed, n_polys=7)
+    s.set_engine(engine)
+    o = to_oracle(s, 32)
+    for k in range(90):
+        s.update(1.0 / 60.0)
+        assert o.update(1.0 / 60.0) == 0
+        if (k + 1) % 30 == 0:
+            a, b = state_acc(s), state_acc(o)
+            assert bits_equal(a, b), f"step {k + 1}: first diff {first_diff(a, b)}"
+    assert s.engine_used == engine
+
+
+@pytest.mark.parametrize("link_iters", [1, 3])
+def test_polygon_wider_than_the_workgroup(link_iters):
+    # a 300-point ring (the SCENE engine picks 256 lanes for 330 bodies, so
+    # lanes take several of its points per edge) squeezed by two small ones
+    s = Solver()
+    s.set_engine(BD_ENGINE_SCENE)
+    s.add_polygon_circle(30.0, (50.0, 50.0), 300, False)
+    s.add_polygon_circle(4.0, (50.0, 78.0), 12, False)
+    s.add_polygon_circle(4.0, (22.0, 50.0), 18, False)
+    s.set_sub_steps(2)
+    s.set_link_iterations(link_iters)
+    run_pair(s, 60, 1.0 / 60.0, check_every=20, engine=BD_ENGINE_SCENE)
 
 
 @pytest.mark.parametrize("tile,wg", [(4096, 512), (1024, 256), (2048, 1024)])
@@ -211,4 +242,4 @@ def test_auto_engine_choice():
     poly = Solver()
     poly.add_polygon_circle(5.0, (50.0, 50.0), 8)
     poly.update(1 / 60)
-    assert poly.engine_used == BD_ENGINE_GLOBAL
+    assert poly.engine_used == BD_ENGINE_SCENE

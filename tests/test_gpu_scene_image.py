@@ -6,7 +6,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from helpers import bits_equal, first_diff, state, to_oracle
+from helpers import bits_equal, first_diff, polygon_scene, state, state_acc, to_oracle
 from hipmem import DevBuf, Graph, Stream
 from bendy2d_amd import SceneImage, Solver, _lib, lib, workloads
 
@@ -43,6 +43,29 @@ def test_image_updates_match_oracle():
         for _ in range(steps):
             assert o.update(dt) == 0
         check_image(img, dev, s, o)
+    for b in (dev, ws):
+        b.free()
+    stream.destroy()
+
+
+def test_polygon_scene_image():
+    # soft polygons, circles, a cloth and static colliders in one image; the
+    # acc that static polygon points gather over the image's updates comes back
+    s = polygon_scene(4010, n_polys=5)
+    o = to_oracle(s, 32)
+    img = SceneImage(s)
+    dev, ws, stream = DevBuf(img.info.image_bytes), DevBuf(img.workspace_bytes), Stream()
+    dev.upload(img.host)
+    for steps in (1, 4, 15):
+        img.update(dev.ptr, ws.ptr, ws.nbytes, DT, steps, stream.h)
+        stream.sync()
+        for _ in range(steps):
+            assert o.update(DT) == 0
+        host = np.zeros_like(img.host)
+        dev.download(host)
+        SceneImage.read(s, host)
+        a, b = state_acc(s), state_acc(o)
+        assert bits_equal(a, b), f"after {steps}: first diff {first_diff(a, b)}"
     for b in (dev, ws):
         b.free()
     stream.destroy()

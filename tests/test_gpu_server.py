@@ -4,7 +4,7 @@ the memory budget forces scenes out to host memory and back."""
 import numpy as np
 import pytest
 
-from helpers import bits_equal, first_diff, state, to_oracle
+from helpers import bits_equal, first_diff, polygon_scene, state, state_acc, to_oracle
 from bendy2d_amd import Server, Solver, workloads
 
 pytestmark = pytest.mark.gpu
@@ -60,6 +60,32 @@ def test_mix_matches_oracle(budget_us):
         # one update per scene and launch: at least as many launches as the
         # longest scene history has updates
         assert st["launches"] >= max(sum(k for k, _ in h) for h in history)
+    srv.close()
+
+
+def test_polygon_scenes_served():
+    # soft polygon scenes among plain ones, over several requests each, with a
+    # one-update time budget so that they finish in uneven launches
+    srv = Server(memory_budget=1 << 30, steps_per_launch=3, launch_budget_us=1)
+    scenes = [polygon_scene(600 + i, n_polys=2 + i) for i in range(4)] + [scene(610), scene(611)]
+    ids, history = [], []
+    for i, s in enumerate(scenes):
+        _, sid = srv.create(s, 4 + i, DT)
+        ids.append(sid); history.append([(4 + i, DT)])
+    for i in (0, 2, 4, 1):
+        srv.advance(ids[i], 5, 1.0 / 120.0)
+        history[i].append((5, 1.0 / 120.0))
+    assert srv.wait_idle(120)
+    done = srv.poll()
+    assert len(done) == 10 and all(d["status"] == 0 for d in done)
+    for i, s in enumerate(scenes):
+        o = to_oracle(s, 32)
+        for steps, dt in history[i]:
+            for _ in range(steps):
+                assert o.update(dt) == 0
+        srv.read_scene(ids[i], s)
+        got, want = state_acc(s), state_acc(o)
+        assert bits_equal(got, want), f"scene {i}: {first_diff(got, want)}"
     srv.close()
 
 

@@ -14,16 +14,18 @@ def scene(kind):
     if kind == "scene":
         workloads.softbody(s, 777, max_side=30)
         engine = BD_ENGINE_SCENE
-    elif kind == "global":
+    elif kind in ("global", "mixed"):
+        # particles, circles, soft polygons and static colliders
         s = _mixed_scene()
-        engine = BD_ENGINE_GLOBAL
+        engine = BD_ENGINE_GLOBAL if kind == "global" else BD_ENGINE_SCENE
+        s.set_engine(engine)
     else:
         workloads.cloth_large(s, n=150)
         engine = BD_ENGINE_TILED
     return s, engine
 
 
-@pytest.mark.parametrize("kind", ["scene", "global", "tiled"])
+@pytest.mark.parametrize("kind", ["scene", "mixed", "global", "tiled"])
 def test_save_mid_run_and_continue(kind):
     s, engine = scene(kind)
     o = to_oracle(s, 32, candidate_mode=True)
