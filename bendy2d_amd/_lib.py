@@ -74,7 +74,7 @@ class BatchConfig(Structure):
 
 class ServerConfig(Structure):
     _fields_ = [("memory_budget", c_uint64), ("steps_per_launch", c_uint32), ("max_active", c_uint32),
-                ("launch", BatchConfig), ("launch_budget_us", c_uint32), ("reserved", c_uint32)]
+                ("launch", BatchConfig), ("launch_budget_us", c_uint32), ("heavy_update_us", c_uint32)]
 
 
 class RequestDone(Structure):
@@ -87,7 +87,8 @@ class ServerStats(Structure):
     _fields_ = [("resident_scenes", c_uint64), ("known_scenes", c_uint64), ("bytes_used", c_uint64),
                 ("bytes_budget", c_uint64), ("launches", c_uint64), ("particle_updates", c_uint64),
                 ("requests_done", c_uint64), ("requests_pending", c_uint64), ("evictions", c_uint64),
-                ("readmissions", c_uint64), ("waiting_admission", c_uint64), ("busy_s", ctypes.c_double)]
+                ("readmissions", c_uint64), ("waiting_admission", c_uint64), ("busy_s", ctypes.c_double),
+                ("heavy_launches", c_uint64)]
 
 
 BD_ORDER_COLOUR = 0
@@ -186,11 +187,12 @@ _SIGS = [
     ("bd_batch_autotune", c_int32, [_S, c_float, c_uint32]),
     ("bd_batch_last_timing", c_int32, [_S, POINTER(c_float), POINTER(c_uint32)]),
     ("bd_batch_cc_stats", c_int32, [_S, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
+    ("bd_batch_cc_fallbacks", c_int32, [_S, POINTER(c_uint64), POINTER(c_uint64)]),
     ("bd_batch_scene_times", c_int32, [_S, c_int64] + [POINTER(ctypes.c_double)] * 4),
     ("bd_ext_copy_calibration", c_int32, [c_uint64, POINTER(c_float)]),
     ("bd_batch_scene_lists", c_int32, [_S, c_int64, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
                                        POINTER(c_float), POINTER(c_float)] + [POINTER(ctypes.c_double)] * 3
-                                      + [POINTER(c_uint32)] * 2),
+                                      + [POINTER(c_uint32)] * 3),
     ("bd_batch_scene_status", c_int32, [_S, c_int64, POINTER(c_float), POINTER(c_uint32), POINTER(c_uint32),
                                         POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32)]),
     ("bd_server_new", _S, [POINTER(ServerConfig)]),

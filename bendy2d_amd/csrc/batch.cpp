@@ -444,6 +444,21 @@ int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64
     return BD_OK;
 }
 
+int32_t bd_batch_cc_fallbacks(bd_batch *b, uint64_t *fallbacks, uint64_t *scenes) {
+    if (!b) return BD_ERR_ARG;
+    size_t n = b->images.size();
+    std::vector<SceneState> st(n);
+    if (b->uploaded && n) {
+        if (hipMemcpy(st.data(), b->states.p, n * sizeof(SceneState), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(BD_ERR_DEVICE, "state readback failed");
+    }
+    uint64_t f = 0, k = 0;
+    for (const SceneState &x : st) { f += x.fallbacks; k += x.fallbacks ? 1 : 0; }
+    if (fallbacks) *fallbacks = f;
+    if (scenes) *scenes = k;
+    return BD_OK;
+}
+
 int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *err, uint32_t *rounds,
                               uint32_t *redos, uint32_t *rebuilds, uint32_t *pairs) {
     if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
@@ -485,7 +500,7 @@ int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *bui
 
 int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t *n_far, uint32_t *max_list,
                              float *s_cell, float *smax, double *t_count, double *t_lists, double *t_sort,
-                             uint32_t *kept, uint32_t *unchecked) {
+                             uint32_t *kept, uint32_t *unchecked, uint32_t *fallbacks) {
     if (!b || id < 0 || (uint64_t)id >= b->images.size()) return BD_ERR_ARG;
     SceneState st{};
     if (b->uploaded && hipMemcpy(&st, b->states.p + id, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess)
@@ -501,6 +516,7 @@ int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t 
     if (t_sort) *t_sort = st.t_sort * tick;
     if (kept) *kept = st.kept;
     if (unchecked) *unchecked = st.unchecked;
+    if (fallbacks) *fallbacks = st.fallbacks;
     return BD_OK;
 }
 

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -139,11 +140,26 @@ constexpr uint32_t kMidList = 256;  // longer candidate lists are sorted by the 
 constexpr uint32_t kSlot = 24;      // normal partners kept from the counting pass (more are gathered again)
 static_assert(2 * kSlot <= 8 * kCandCap, "count-pass slots must fit in the trajectory array");
 constexpr uint32_t kNone = 0xffffffffu;
+// Failed attempts at a sweep after which it runs sequentially instead.
+constexpr int kFallbackAttempt = 4;
+// Sweeps after such a fallback that run sequentially without trying lists.
+constexpr uint32_t kFallbackStreak = 16;
 // A list is rebuilt before a sweep once some circle drifted this fraction of
 // its skin from its list-build position (relative to the scene's mean shift).
 constexpr float kDriftRebuild = 1.0f;
 
+// FbGrid: bucket heads (u32, tmask + 1 of them), per-circle next links
+// (u16) and a candidate buffer of cap entries (u16): in LDS when the scene
+// leaves room, else in the workspace.  A circle's bucket is always that of
+// its current position, so it is not stored.
+struct FbGrid {
+    uint32_t *head;
+    uint16_t *next, *cand;
+    uint32_t tmask, cap;
+};
+
 struct CCShared {
+    CCWork work;
     uint16_t out[kOutCap];   // outlier circles of the current list
     uint32_t hist[33];       // skins by log2(skin / base)
     uint32_t nout;
@@ -153,10 +169,20 @@ struct CCShared {
     uint32_t cnt3[3];        // worklist lengths, rotating (see the round loop)
     uint32_t poly_last[2];   // polygon pairs: 1 + the last point of the other polygon that
                              // hit the current edge, two slots used alternately
+    // LDS the sequential fallback may use for its grid: the scene's previous
+    // positions when they are held in LDS (parked in q_global meanwhile), or
+    // what the positions leave of the launch's LDS
+    struct Fallback {
+        uint8_t *lds;
+        uint32_t lds_bytes, nq;
+        float2 *q_lds, *q_global;
+    } fb;
+    FbGrid grid;             // the fallback's grid (see sequential_sweep)
+    uint32_t fb_streak;      // sweeps left that go straight to the fallback
     // statistics of the scene during a launch, written by lane 0 only (in LDS
     // rather than in every lane's registers)
     struct Counts {
-        uint32_t rounds, redos, rebuilds, kept, unchecked, n_out, n_far, max_list;
+        uint32_t rounds, redos, rebuilds, kept, unchecked, n_out, n_far, max_list, fallbacks;
         float smax;
         uint64_t t_build, t_count, t_lists, t_sort, t_links, t_sweep, t_bodies;
     } cnt;
@@ -175,7 +201,7 @@ __device__ __forceinline__ Drift drift_summary(const float2 *P, uint32_t Np, uin
                                                float *s_red9, float smin) {
     // skins change only when the list is built, which keeps their minimum
     Drift d{0.0f, 0.0f, 0.0f, INFINITY, -INFINITY, INFINITY, -INFINITY, smin, 0.0f};
-#pragma unroll 2
+#pragma unroll 1
     for (uint32_t c = threadIdx.x; c < Nc; c += WG) {
         const float2 p = P[Np + c], r0 = w.ref[c];
         if (!(fabsf(p.x) <= 3.0e38f) || !(fabsf(p.y) <= 3.0e38f)) continue;  // touches nothing
@@ -263,7 +289,7 @@ __device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, floa
 // (the test is symmetric) and goes into the first one's list.  MODE 0 counts
 // into cnt, MODE 1 appends at the lists' cursors.  Returns the far outliers.
 template <int WG, int MODE>
-__device__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, const float *crad, const uint16_t *outs,
+__device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, const float *crad, const uint16_t *outs,
                                     uint32_t n_out, float rmax, float s_cell, double h, double inv_h,
                                     uint32_t tmask, uint32_t *cursor) {
     auto add = [&](uint32_t c, uint32_t o) {
@@ -493,6 +519,273 @@ struct CCStats {
     float s_cell, smin;
 };
 
+// Sequential fallback of the circle-pair sweep, for a sweep whose candidate
+// lists keep failing (circles pushed far within the sweep, lists beyond the
+// workspace): the reference's (i, j) loop (src/solver.rs:168-177) row by
+// row, which needs no list made in advance.  Run by the first wavefront
+// alone; the caller holds the others at a barrier.
+//
+// The circles sit in a hashed grid of cell size h = 2 r_max + M, each in
+// the bucket of the cell its current position is in (a push that moves a
+// circle to another bucket moves it in the grid).  Row i gathers every
+// j > i closer than r_i + r_j + M to circle i (nine lanes walk the nine
+// buckets of the 3x3 cells around it); the wavefront ranks them, and one
+// lane runs the pairs in ascending j.  Only circle i moves within its row
+// besides each partner at its own turn, so the gather is complete while
+// circle i stays within M/2 of where it was gathered from; when it moves
+// further the rest of the row is gathered again.  The halved margin leaves
+// room for f32 rounding of the distance tests.
+//
+constexpr uint32_t kNone16 = 0xffffu;
+
+template <int WG>
+__device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r,
+                                              float runi, float rmax, const FbGrid *gp, uint32_t *s_fb) {
+    const FbGrid g = *gp;
+    const uint32_t lane = threadIdx.x & 63u;
+    const float rm = fmaxf(rmax, 1e-3f);
+    const float M = 0.5f * rm;
+    const double inv_h = 1.0 / (2.0 * (double)rm + (double)M);
+    uint32_t *const head = g.head;
+    uint16_t *const next = g.next, *const cand = g.cand;
+    const uint32_t tmask = g.tmask;
+    auto radius = [=](uint32_t c) { return uni_r ? runi : crad[c]; };
+    auto finite2 = [](float2 p) { return fabsf(p.x) <= 3.0e38f && fabsf(p.y) <= 3.0e38f; };
+    auto bucket = [=](float2 p) {
+        return finite2(p) ? (cell_hash(scell(p.x, inv_h), scell(p.y, inv_h)) & tmask) : kNone;
+    };
+    auto wave_sync = []() { __threadfence_block(); __builtin_amdgcn_wave_barrier(); };
+    auto link = [=](uint32_t c, uint32_t k) {  // c into bucket k (lane 0 alone)
+        const uint32_t h0 = head[k];
+        next[c] = h0 == kNone ? kNone16 : (uint16_t)h0;
+        head[k] = c;
+    };
+    auto unlink = [=](uint32_t c, uint32_t k) {  // c out of bucket k (lane 0 alone; lists are acyclic)
+        uint32_t x = head[k];
+        if (x == c) {
+            head[k] = next[c] == kNone16 ? kNone : next[c];
+        } else {
+#pragma unroll 1
+            for (uint32_t guard = 0; x != kNone && guard < Nc; guard++) {
+                const uint32_t y = next[x];
+                if (y == c) { next[x] = next[c]; break; }
+                x = y == kNone16 ? kNone : y;
+            }
+        }
+    };
+#pragma unroll 1
+    for (uint32_t b = lane; b <= tmask; b += 64) head[b] = kNone;
+    wave_sync();
+#pragma unroll 1
+    for (uint32_t c = lane; c < Nc; c += 64) {
+        const uint32_t k = bucket(P[Np + c]);
+        if (k != kNone) {
+            const uint32_t h0 = atomicExch(&head[k], c);
+            next[c] = h0 == kNone ? kNone16 : (uint16_t)h0;
+        }
+    }
+    wave_sync();
+#pragma unroll 1
+    for (uint32_t i = 0; i + 1 < Nc; i++) {
+        if (!finite2(P[Np + i])) continue;  // touches nothing (every comparison with it is false)
+        const float ri = radius(i);
+        uint32_t jlast = i;
+#pragma unroll 1
+        for (uint32_t pass = 0; pass < Nc; pass++) {
+            const float2 base = P[Np + i];
+            if (lane == 0) s_fb[0] = 0u;
+            wave_sync();
+            // gather: lanes 0-8 walk the distinct buckets of the 3x3 cells
+            const int cx0 = scell(base.x, inv_h), cy0 = scell(base.y, inv_h);
+            const uint32_t kk = cell_hash(cx0 + (int)(lane % 3u) - 1, cy0 + (int)(lane / 3u) - 1) & tmask;
+            bool dup = false;
+#pragma unroll 1
+            for (int l2 = 0; l2 < 8; l2++) {
+                const uint32_t other = (uint32_t)__shfl(kk, l2, 64);  // every lane takes part
+                dup |= (uint32_t)l2 < lane && other == kk;
+            }
+            if (lane < 9u && !dup) {
+                uint32_t j = head[kk];
+#pragma unroll 1
+                for (uint32_t guard = 0; j != kNone && guard < Nc; guard++) {
+                    const uint32_t nx = next[j];
+                    if (j > jlast) {
+                        const float2 pj = P[Np + j];
+                        const float dx = base.x - pj.x, dy = base.y - pj.y, lim = (ri + radius(j)) + M;
+                        if (dx * dx + dy * dy < lim * lim) {
+                            const uint32_t at = atomicAdd(&s_fb[0], 1u);
+                            if (at < g.cap) cand[at] = (uint16_t)j;
+                        }
+                    }
+                    j = nx == kNone16 ? kNone : nx;
+                }
+            }
+            wave_sync();
+            const uint32_t n = s_fb[0];
+            const bool fits = n <= g.cap && n <= kMidList;
+            if (fits) {
+                sort_by_wave(cand, n);  // ascending j: the reference's order
+                wave_sync();
+                // the lanes test the next 64 candidates against circle i where
+                // it is now; those before the first that overlaps are no-ops
+                // (the reference's test, false), so one lane runs only the
+                // pairs that overlap, in order
+                uint32_t again = 0u;
+#pragma unroll 1
+                for (uint32_t k0 = 0; k0 < n;) {
+                    const float2 a = P[Np + i];
+                    const uint32_t k = k0 + lane;
+                    bool t = false;
+                    if (k < n) {
+                        const uint32_t j = cand[k];
+                        const float2 pj = P[Np + j];
+                        const float dx = a.x - pj.x, dy = a.y - pj.y, rs = ri + radius(j);
+                        t = dx * dx + dy * dy < rs * rs;
+                    }
+                    const uint64_t hit = __ballot(t);
+                    if (!hit) { k0 += 64u; continue; }
+                    const uint32_t kf = k0 + (uint32_t)__builtin_ctzll(hit);
+                    const uint32_t j = cand[kf];
+                    if (lane == 0) {
+                        float2 ai = a;
+                        const float2 b0 = P[Np + j];
+                        float2 b = b0;
+                        float qi, qj;
+                        uint32_t moved = 0u;
+                        if (solve_circle_pair(ai.x, ai.y, b.x, b.y, ri, radius(j), qi, qj)) {
+                            P[Np + i] = ai;
+                            P[Np + j] = b;
+                            const uint32_t ob = bucket(b0), nb = bucket(b);  // b0 is finite: it was gathered
+                            if (nb != ob) {
+                                unlink(j, ob);
+                                if (nb != kNone) link(j, nb);
+                            }
+                            const float gx = ai.x - base.x, gy = ai.y - base.y;
+                            moved = !(gx * gx + gy * gy <= 0.25f * M * M) ? 1u : 0u;
+                        }
+                        s_fb[1] = moved;
+                    }
+                    wave_sync();
+                    jlast = j;
+                    k0 = kf + 1u;
+                    if (s_fb[1]) { again = 1u; break; }
+                }
+                if (!again) break;  // the row is done
+                continue;           // gather the rest of the row again
+            }
+            wave_sync();
+            if (lane == 0) {
+                uint32_t again = 0u;
+                float2 a = P[Np + i];
+#pragma unroll 1
+                for (uint32_t k = 0; k < n; k++) {
+                    uint32_t j;
+                    {
+                        // too many to rank: the next j above jlast within reach,
+                        // by walking the nine buckets again (lane 0 alone)
+                        j = kNone;
+#pragma unroll 1
+                        for (int q = 0; q < 9; q++) {
+                            const uint32_t kq = cell_hash(cx0 + q % 3 - 1, cy0 + q / 3 - 1) & tmask;
+                            uint32_t x = head[kq];
+#pragma unroll 1
+                            for (uint32_t guard = 0; x != kNone && guard < Nc; guard++) {
+                                const uint32_t nx = next[x];
+                                if (x > jlast && x < j) {
+                                    const float2 px = P[Np + x];
+                                    const float dx = base.x - px.x, dy = base.y - px.y;
+                                    const float lim = (ri + radius(x)) + M;
+                                    if (dx * dx + dy * dy < lim * lim) j = x;
+                                }
+                                x = nx == kNone16 ? kNone : nx;
+                            }
+                        }
+                        if (j == kNone) break;
+                    }
+                    jlast = j;
+                    const float2 b0 = P[Np + j];
+                    float2 b = b0;
+                    float qi, qj;
+                    if (solve_circle_pair(a.x, a.y, b.x, b.y, ri, radius(j), qi, qj)) {
+                        P[Np + i] = a;
+                        P[Np + j] = b;
+                        const uint32_t ob = bucket(b0), nb = bucket(b);  // b0 is finite: it was gathered
+                        if (nb != ob) {
+                            unlink(j, ob);
+                            if (nb != kNone) link(j, nb);
+                        }
+                        const float gx = a.x - base.x, gy = a.y - base.y;
+                        if (!(gx * gx + gy * gy <= 0.25f * M * M)) { again = 1u; break; }
+                    }
+                }
+                s_fb[1] = again;
+                s_fb[2] = jlast;
+            }
+            wave_sync();
+            jlast = s_fb[2];
+            if (!s_fb[1]) break;
+        }
+    }
+    wave_sync();
+}
+
+// The sequential sweep for the whole workgroup: the first wavefront runs it,
+// then every circle's skin need follows from how far the sweep moved it, and
+// the lists are marked stale.  Its grid goes into LDS when the scene leaves
+// room there (previous positions held in LDS are parked in global memory
+// for the sweep), else into the circle-pair workspace.
+template <int WG>
+__device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r,
+                                               float runi, float rmax, uint32_t T, const CCWork &w, CCShared *sh,
+                                               float &sbase, bool &list_valid) {
+    const int tid = threadIdx.x;
+    const CCShared::Fallback fb = sh->fb;
+    for (uint32_t c = tid; c < Nc; c += WG) w.save[c] = P[Np + c];
+    if (fb.q_lds)
+        for (uint32_t j = tid; j < fb.nq; j += WG) fb.q_global[j] = fb.q_lds[j];
+    __syncthreads();
+    // heads 4 B per bucket, next links 2 B per circle, 256 candidates
+    FbGrid &g = sh->grid;
+    const uint64_t fixed = 2ull * Nc + 2ull * kMidList;
+    uint32_t tb = 0;
+    if (fixed + 4ull * 64 <= fb.lds_bytes) {
+        tb = 64;
+        while (tb < 32768u && tb < 2 * Nc && fixed + 4ull * 2 * tb <= fb.lds_bytes) tb <<= 1;
+    }
+    if (tb) {
+        uint8_t *at = fb.lds;
+        g.head = (uint32_t *)at; at += 4ull * tb;
+        g.next = (uint16_t *)at; at += 2ull * Nc;
+        g.cand = (uint16_t *)at;
+        g.tmask = tb - 1;
+        g.cap = kMidList;
+    } else {
+        // workspace: T + 1 heads (start), next links in key, candidates in stamp
+        g.head = w.start;
+        g.next = (uint16_t *)w.key;
+        g.cand = (uint16_t *)w.stamp;
+        g.tmask = T - 1;
+        g.cap = 2 * Nc;
+    }
+    __syncthreads();  // the grid's description, written by every thread alike, before the first wavefront reads it
+    if (tid < 64) sequential_sweep<WG>(P, Np, Nc, crad, uni_r, runi, rmax, &sh->grid, sh->cnt3);
+    __syncthreads();
+    if (fb.q_lds)
+        for (uint32_t j = tid; j < fb.nq; j += WG) fb.q_lds[j] = fb.q_global[j];
+    for (uint32_t c = tid; c < Nc; c += WG) {
+        const float2 p = P[Np + c], q = w.save[c];
+        const float dx = p.x - q.x, dy = p.y - q.y;
+        const float d = sqrtf(dx * dx + dy * dy) * (1.0f + 0x1p-20f);
+        w.need[c] = (d <= 3.0e38f) ? fmaxf(1.5f * d, 0.75f * w.need[c]) : 0.0f;
+    }
+    if (tid == 0) sh->cnt.fallbacks++;
+    list_valid = false;
+    // a base skin driven below its floor by lists that did not fit starts
+    // over from the usual one
+    sbase = fmaxf(sbase, 0.05f * fmaxf(rmax, 1e-3f));
+    __syncthreads();
+}
+
 // One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
 // candidate pairs run in rounds; pair (c, o), c < o, is ready when o is the
 // smallest unprocessed candidate of c and c that of o, so every earlier pair
@@ -515,8 +808,17 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                                  float &sbase, bool &list_valid, CCStats &cs) {
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
-    for (int attempt = 0;; attempt++) {
-        if (attempt > 20) return 2u;
+    // a scene whose lists just failed goes on without them for a few sweeps
+    bool fallback = false;
+    for (int attempt = uni(sh->fb_streak) != 0u ? kFallbackAttempt : 0;; attempt++) {
+        if (fallback || attempt >= kFallbackAttempt) {
+            // the lists keep failing: the sequential sweep, from the sweep's
+            // start positions (P holds them: every redo restores them)
+            fallback_sweep<WG>(P, Np, Nc, crad, uni_r, runi, rmax, T, w, sh, sbase, list_valid);
+            if (tid == 0) sh->fb_streak = sh->fb_streak ? sh->fb_streak - 1u : kFallbackStreak;
+            __syncthreads();
+            return 0u;
+        }
         bool rebuild = !list_valid;
         float2 u = make_float2(0.0f, 0.0f);
         if (!rebuild) {
@@ -639,7 +941,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 }
                 __syncthreads();
                 list_valid = false;
-                if (sbase < 1e-4f * fmaxf(rmax, 1e-3f)) return 1u;
+                if (sbase < 1e-4f * fmaxf(rmax, 1e-3f)) fallback = true;
                 continue;
             }
             // lists: normal partners first (from the slot, or gathered again when
@@ -676,7 +978,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 longest = max(longest, n);
                 if (n <= 1u) continue;
                 if (n <= 8u) sort_in_registers<8>(my, n);
-                else if (n <= 16u) sort_in_registers<16>(my, n);
+                else if (WG < 1024 && n <= 16u) sort_in_registers<16>(my, n);
                 // 32 registers more than a 1,024-lane workgroup's 128 allow
                 // without spilling: there such lists go to a wavefront
                 else if (WG < 1024 && n <= 32u) sort_in_registers<(WG < 1024 ? 32 : 16)>(my, n);
@@ -855,10 +1157,47 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
     }
 }
 
+// Polygon pairs of one scene (see run_scene), kept out of line: a rare phase
+// whose registers need not weigh on the rest of the kernel.
+template <int WG>
+__device__ __noinline__ void poly_pairs(float2 *P, const uint4 *ptab, const float2 *ctr, uint32_t npoly,
+                                        uint32_t *poly_last) {
+    const int tid = threadIdx.x;
+    if (tid == 0) { poly_last[0] = 0u; poly_last[1] = 0u; }
+    __syncthreads();
+    uint32_t turn = 0;
+    for (uint32_t i = 0; i < npoly; i++)
+        for (uint32_t j = i + 1; j < npoly; j++)
+            for (uint32_t dir = 0; dir < 2; dir++) {
+                const uint32_t si = dir ? j : i, oi = dir ? i : j;
+                const uint4 ts = ptab[si], to = ptab[oi];
+                const float2 sc = ctr[si], oc = ctr[oi];
+                for (uint32_t e = 0; e < ts.y; e++, turn++) {
+                    const uint32_t ia = ts.x + e, ib = ts.x + (e + 1 == ts.y ? 0u : e + 1);
+                    const float2 pa = P[ia], pb = P[ib];
+                    uint32_t mine = 0;
+                    float2 na = pa, nb = pb;
+                    for (uint32_t k = tid; k < to.y; k += WG) {
+                        float2 a2, b2, o2;
+                        if (resolve_line(sc, pa, pb, P[to.x + k], oc, a2, b2, o2)) {
+                            P[to.x + k] = o2;
+                            na = a2; nb = b2; mine = k + 1;
+                        }
+                    }
+                    uint32_t *slot = &poly_last[turn & 1u];
+                    if (mine) atomicMax(slot, mine);
+                    __syncthreads();
+                    if (mine && mine == *slot) { P[ia] = na; P[ib] = nb; }
+                    if (tid == 0) poly_last[(turn & 1u) ^ 1u] = 0u;
+                    __syncthreads();
+                }
+            }
+}
+
 // One scene for `steps` updates.  MODE 2 keeps positions and previous
 // positions in LDS (s_lds), MODE 1 positions only, MODE 0 works on the scene
 // chunk in global memory.
-template <int WG, int MODE>
+template <int WG, int MODE, bool SEQ>
 __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint32_t steps, SceneState *st,
                                           float2 *s_lds, ColliderStage *s_col, float *s_red, uint32_t *s_u32,
                                           CCShared *s_cc, uint32_t *s_wl, uint32_t lds_bytes) {
@@ -908,12 +1247,23 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     if (links_lds)
         for (uint32_t l = tid; l < d.L; l += WG) s_links[l] = links[l];
     const bool pins = (d.flags & kFlagPins) != 0;
+    if (tid == 0) {
+        CCShared::Fallback &f = s_cc->fb;
+        if constexpr (MODE == 2) {
+            f.lds = (uint8_t *)Q; f.lds_bytes = 8u * N; f.q_lds = Q; f.q_global = gprev; f.nq = N;
+        } else {
+            const uint32_t used = MODE == 1 ? 8u * N : 0u;
+            f.lds = (uint8_t *)(s_lds) + used; f.lds_bytes = lds_bytes > used ? lds_bytes - used : 0u;
+            f.q_lds = nullptr; f.q_global = nullptr; f.nq = 0u;
+        }
+    }
     __syncthreads();
 
     // circle-pair workspace
     char *ws = d.ws_base ? (char *)d.ws_base : base + d.ws_off;
     const CCLayout cl = cc_layout(Nc, d.T);
-    CCWork w;
+    CCWork &w = s_cc->work;
+    if (tid == 0) {
     w.ref = (float2 *)(ws + cl.ref); w.save = (float2 *)(ws + cl.save);
     w.path = (float *)(ws + cl.path); w.skin = (float *)(ws + cl.skin); w.need = (float *)(ws + cl.need);
     w.cnt = (uint32_t *)(ws + cl.cnt); w.off = (uint32_t *)(ws + cl.off); w.cand = (uint16_t *)(ws + cl.cand);
@@ -921,19 +1271,22 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     w.head = (uint32_t *)(ws + cl.head); w.stamp = (uint32_t *)(ws + cl.stamp); w.wl = (uint32_t *)(ws + cl.wl);
     w.key = (uint32_t *)(ws + cl.key); w.rec = (float4 *)(ws + cl.rec); w.start = (uint32_t *)(ws + cl.start);
     w.fill = (uint32_t *)(ws + cl.fill); w.items = (uint32_t *)(ws + cl.items);
+    }
+    __syncthreads();
     if (Nc >= 2 && !st->ws_ready) {
         for (uint32_t c = tid; c < Nc; c += WG) w.need[c] = 0.0f;
         __syncthreads();
     }
 
     float skin = uni(st->skin);
-    bool list_valid = uni(st->list_valid) != 0;
+    bool list_valid = uni((uint32_t)st->list_valid) != 0;
     uint32_t err = uni(st->err);
     CCStats cs{};
     cs.pairs = uni(st->pairs);
     cs.since_build = uni(st->since_build);
     cs.s_cell = uni(st->s_cell); cs.smin = uni(st->smin);
     if (tid == 0) {
+        s_cc->fb_streak = 0u;
         CCShared::Counts &k = s_cc->cnt;
         k = CCShared::Counts{};
         k.n_out = st->n_out; k.n_far = st->n_far; k.max_list = st->max_list; k.smax = st->smax;
@@ -944,8 +1297,11 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
 
     const uint64_t t_scene0 = wall_clock64();
     uint32_t ran = 0;
+    // a scene stopped by its time budget inside an update resumes at the
+    // substep after the last one it ran
+    uint32_t sub0 = uni((uint32_t)st->sub_phase), resume = 0u;
     for (uint32_t step = 0; step < steps; step++) {
-        for (uint32_t sub = 0; sub < d.sub_steps; sub++) {
+        for (uint32_t sub = sub0; sub < d.sub_steps; sub++) {
             // ---- links: dependency layers, k sweeps
             const uint64_t tl0 = wall_clock64();
             // polygon centres (Polygon::calc_center, src/polygon.rs:231-237) as
@@ -970,7 +1326,8 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                 if (npoly && it + 1 == d.link_iters) poly_centres();
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = loff[L], hi = loff[L + 1];
-                    // not unrolled (measured faster than unrolling by 2, 4 or 8:
+                    // not unrolled (measured faster than unrolling by 2, 4 or 8,
+                    // and than loading the next layer's first record early:
                     // register pressure of the whole persistent kernel); the
                     // thread's next record loads while this one is solved
                     uint32_t l = lo + tid;
@@ -1004,8 +1361,11 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             if (Nc >= 2 && !err)
             {
                 const uint64_t t0 = wall_clock64();
-                err = circle_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl, skin,
-                                       list_valid, cs);
+                if constexpr (SEQ)
+                    fallback_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, skin, list_valid);
+                else
+                    err = circle_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl,
+                                           skin, list_valid, cs);
                 if (tid == 0) s_cc->cnt.t_sweep += wall_clock64() - t0;
             }
             // ---- polygon pairs (src/solver.rs:178-187): Polygon::solve_polygon
@@ -1014,38 +1374,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             // ends as they were when its turn came, its own position and the two
             // centres), so they run across the workgroup; the last point that
             // hits (highest index) decides where the edge's ends go.
-            if (npoly >= 2) {
-                if (tid == 0) { s_cc->poly_last[0] = 0u; s_cc->poly_last[1] = 0u; }
-                __syncthreads();
-                uint32_t turn = 0;
-                for (uint32_t i = 0; i < npoly; i++)
-                    for (uint32_t j = i + 1; j < npoly; j++)
-                        for (uint32_t dir = 0; dir < 2; dir++) {
-                            const uint32_t si = dir ? j : i, oi = dir ? i : j;
-                            const uint4 ts = ptab[si], to = ptab[oi];
-                            const float2 sc = ctr[si], oc = ctr[oi];
-                            for (uint32_t e = 0; e < ts.y; e++, turn++) {
-                                const uint32_t ia = ts.x + e, ib = ts.x + (e + 1 == ts.y ? 0u : e + 1);
-                                const float2 pa = P[ia], pb = P[ib];
-                                uint32_t mine = 0;
-                                float2 na = pa, nb = pb;
-                                for (uint32_t k = tid; k < to.y; k += WG) {
-                                    float2 a2, b2, o2;
-                                    if (resolve_line(sc, pa, pb, P[to.x + k], oc, a2, b2, o2)) {
-                                        P[to.x + k] = o2;
-                                        na = a2; nb = b2; mine = k + 1;
-                                    }
-                                }
-                                uint32_t *slot = &s_cc->poly_last[turn & 1u];
-                                if (mine) atomicMax(slot, mine);
-                                __syncthreads();
-                                if (mine && mine == *slot) { P[ia] = na; P[ib] = nb; }
-                                // the other slot was last read before the previous barrier
-                                if (tid == 0) s_cc->poly_last[(turn & 1u) ^ 1u] = 0u;
-                                __syncthreads();
-                            }
-                        }
-            }
+            if (npoly >= 2) poly_pairs<WG>(P, ptab, ctr, npoly, s_cc->poly_last);
             // ---- bodies: colliders -> bounds -> Verlet
             const uint64_t tq0 = wall_clock64();
             const int4 *hdr = staged ? s_col->hdr : ghdr;
@@ -1077,11 +1406,21 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             acc = false;
             __syncthreads();
             if (tid == 0) s_cc->cnt.t_bodies += wall_clock64() - tq0;
+            if (d.budget && sub + 1 < d.sub_steps) {
+                // stop after a substep once the budget is spent: the state
+                // between substeps is whole, and the next launch resumes
+                // there (one clock read, shared, so the workgroup leaves
+                // together)
+                if (tid == 0) s_cc->halt = (wall_clock64() - t_scene0 > (uint64_t)d.budget) ? 1u : 0u;
+                __syncthreads();
+                if (uni(s_cc->halt)) { resume = sub + 1; break; }
+            }
         }
+        if (resume) break;
+        sub0 = 0u;
         ran = step + 1;
         if (d.budget && ran < steps) {
-            // stop after a whole update once the budget is spent (one clock read,
-            // shared, so the whole workgroup leaves together)
+            // stop after a whole update once the budget is spent
             if (tid == 0) s_cc->halt = (wall_clock64() - t_scene0 > (uint64_t)d.budget) ? 1u : 0u;
             __syncthreads();
             if (uni(s_cc->halt)) break;
@@ -1122,7 +1461,9 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         st->t_sort += k.t_sort;
         st->kept += k.kept;
         st->unchecked += k.unchecked;
+        st->fallbacks += k.fallbacks;
         st->last_steps = ran;
+        st->sub_phase = (uint16_t)resume;
     }
     __syncthreads();
 }
@@ -1148,12 +1489,18 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
         const SceneDesc *__restrict__ d = descs + si;
         const uint32_t steps = d->steps < launch_steps ? d->steps : launch_steps;
         if (steps == 0 || d->N == 0) continue;
-        if (16ull * d->N <= lds_bytes)
-            run_scene<WG, 2>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl, lds_bytes);
+        if (d->flags & kFlagSequential)  // diagnostic: every sweep sequential, state in global memory
+            run_scene<WG, 0, true>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl,
+                                   lds_bytes);
+        else if (16ull * d->N <= lds_bytes)
+            run_scene<WG, 2, false>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl,
+                                    lds_bytes);
         else if (8ull * d->N <= lds_bytes)
-            run_scene<WG, 1>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl, lds_bytes);
+            run_scene<WG, 1, false>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl,
+                                    lds_bytes);
         else
-            run_scene<WG, 0>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl, lds_bytes);
+            run_scene<WG, 0, false>(d, steps, states + d->state_slot, s_lds, &s_col, s_red, s_u32, &s_cc, s_wl,
+                                    lds_bytes);
     }
 }
 
@@ -1326,6 +1673,10 @@ int build_scene_image(const HostScene &s, const Plan &p, SceneImage &img, std::s
         put(d.pl_off, pl.data(), 16ull * d.nplanes);
     }
     d.flags = pins ? kFlagPins : 0u;
+    // diagnostic switch: every circle-pair sweep by the sequential fallback
+    // (its results are the same; tests use it to run that path on any scene)
+    if (const char *e = std::getenv("BD_SCENE_SEQUENTIAL_SWEEP"))
+        if (e[0] == '1') d.flags |= kFlagSequential;
     img.acc_pending = s.any_initial_acc();
     d.rmax = 0.0f;
     for (float r : s.c_r) d.rmax = std::max(d.rmax, r);

@@ -44,8 +44,9 @@ __global__ void __launch_bounds__(WG) k_tiled_pass(const float2 *__restrict__ sr
     const uint32_t t = xcd_tile(blockIdx.x, ntiles);
     const TileInfo ti = tiles[t];
     const int tid = threadIdx.x;
-    for (uint32_t r = tid; r < ti.int_count; r += WG) s_reg[r] = src[ti.int_start + r];
-    for (uint32_t h = tid; h < ti.halo_count; h += WG) s_reg[ti.int_count + h] = src[halo[ti.halo_off + h]];
+    for (uint32_t r = tid; r < ti.int_count; r += WG) s_reg[region_slot(r)] = src[ti.int_start + r];
+    for (uint32_t h = tid; h < ti.halo_count; h += WG)
+        s_reg[region_slot(ti.int_count + h)] = src[halo[ti.halo_off + h]];
     __syncthreads();
     const uint32_t *lo = loff + ti.loff_off;
     const LinkS *lk = links + ti.link_off;
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(WG) k_tiled_pass(const float2 *__restrict__ sr
     if (fuse) {
         for (uint32_t r = tid; r < ti.int_count; r += WG) {
             const uint32_t u = ti.int_start + r;
-            float2 p = s_reg[r];
+            float2 p = s_reg[region_slot(r)];
             if (!(pinmask && ((pinmask[u >> 5] >> (u & 31)) & 1u))) {
                 float2 q = prev[u];
                 particle_body(p, q, bp, colhdr, colgeo, planes);
@@ -73,7 +74,7 @@ __global__ void __launch_bounds__(WG) k_tiled_pass(const float2 *__restrict__ sr
             dst[u] = p;
         }
     } else {
-        for (uint32_t r = tid; r < ti.int_count; r += WG) dst[ti.int_start + r] = s_reg[r];
+        for (uint32_t r = tid; r < ti.int_count; r += WG) dst[ti.int_start + r] = s_reg[region_slot(r)];
     }
 }
 

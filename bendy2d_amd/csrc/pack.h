@@ -61,17 +61,21 @@ constexpr uint32_t kFlagPins = 1u;        // some particle is pinned
 constexpr uint32_t kFlagAcc = 2u;         // initial circle acc pending (first substep)
 constexpr uint32_t kFlagLds = 4u;         // positions held in LDS
 constexpr uint32_t kFlagUniformR = 8u;    // all circles share one radius (runi)
+constexpr uint32_t kFlagSequential = 16u; // every circle-pair sweep by the sequential fallback
 
 // Per-scene state that persists between launches (written by the kernel).
 struct SceneState {
     float skin;          // base candidate margin of the circle-pair schedule
-    uint32_t list_valid; // candidate list built and still usable
+    uint8_t list_valid;  // candidate list built and still usable
+    uint8_t ws_ready;    // per-circle skin state initialised
+    uint16_t sub_phase;  // substeps of the current update already run (a launch's time budget
+                         // stopped the scene inside an update)
     uint32_t err;        // nonzero: the circle-pair schedule failed
     uint32_t rounds;     // schedule rounds executed (statistics)
     uint32_t redos;      // sweeps redone with larger skins
     uint32_t rebuilds;   // candidate lists built
     uint32_t pairs;      // candidate pairs in the last list
-    uint32_t ws_ready;   // per-circle skin state initialised
+    uint32_t fallbacks;  // sweeps run by the sequential fallback (statistics)
     uint64_t t_links, t_build, t_sweep, t_bodies;  // phase times, device wall-clock ticks (statistics)
     uint32_t since_build;  // sweeps since the candidate list was built
     // the last candidate list (statistics): outliers, of them searched against
@@ -81,7 +85,7 @@ struct SceneState {
     uint64_t t_count, t_lists, t_sort;  // list build split: counts, lists, sorting (ticks)
     uint32_t kept;  // sweeps kept although circles left their skins (no missed pair touched)
     uint32_t unchecked;  // sweeps redone because too many circles left their skins, or too far, to check
-    uint32_t last_steps;  // updates the scene's last launch ran
+    uint32_t last_steps;  // updates the scene's last launch completed
     float smin;           // smallest skin of the last candidate list
 };
 static_assert(sizeof(SceneState) <= 128, "SceneState must fit the single-scene device blob");

@@ -141,7 +141,7 @@ int image_update(const bd_scene_image_info *info, void *image, void *ws, uint64_
 
 // Workspace of the allocation-free update: one array per device, holding the
 // circle-pair workspace of any scene an image can hold (16,384 circles need
-// 6.7 MB).  It lives until the process ends.  Host code never uses the
+// 6.4 MiB).  It lives until the process ends.  Host code never uses the
 // array's name as an address: scratch_address() asks the runtime for the
 // device's copy once per device.
 constexpr uint64_t kScratchBytes = 8ull << 20;

@@ -8,6 +8,13 @@
 // descriptor table, each with min(remaining, steps_per_launch) updates; run one
 // persistent batched launch; retire requests whose steps are done.  Requests
 // submitted during a launch join the next one.
+//
+// A scene whose updates take far longer than the launch budget (one update
+// cannot be cut short: a pile whose circles move several diameters per
+// substep, swept sequentially, can take a second) would hold every launch it
+// joins.  Such heavy scenes run in launches of their own on a second stream,
+// beside the others: the scheduler starts one when none is in flight and
+// retires it when its stream is done, while ordinary launches go on.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -95,6 +102,9 @@ struct ServedScene {
     uint64_t last_used = 0;
     uint64_t updates_run = 0;
     bool failed = false;            // its circle-pair schedule failed: requests finish with BD_ERR_COLLISION
+    double ms_per_update = 0.0;     // device time per update, recent launches (phase timers)
+    uint64_t ticks_seen = 0;        // the phase timers' total when last read
+    bool heavy_inflight = false;    // in the heavy launch now running
 };
 
 }  // namespace
@@ -102,6 +112,12 @@ struct ServedScene {
 struct bd_server {
     bd_server_config cfg{};
     BatchCore core;
+    BatchCore heavy;                     // launches of heavy scenes, on their own stream
+    std::vector<ServedScene *> heavy_launch;
+    bool heavy_busy = false;
+    double heavy_t_start = 0.0;
+    double heavy_ms = 100.0;             // a scene above this device time per update is heavy
+    double tick_s = 1e-8;                // seconds per device wall-clock tick
     Arena arena;
     DevBuf<SceneState> slots;
     std::vector<uint32_t> free_slots;
@@ -120,7 +136,7 @@ struct bd_server {
     std::string fatal;
 
     // statistics (scheduler thread; read under mu)
-    uint64_t launches = 0, pu = 0, n_done = 0, evictions = 0, readmissions = 0, waiting = 0;
+    uint64_t launches = 0, pu = 0, n_done = 0, evictions = 0, readmissions = 0, waiting = 0, heavy_launches = 0;
     double busy_s = 0.0;
     uint32_t budget_ticks = 0;           // per-scene launch budget in device wall-clock ticks
     std::vector<SceneState> readback;    // states of the last launch's slot range
@@ -194,6 +210,102 @@ static bool admit(bd_server *S, ServedScene &sc, const std::vector<ServedScene *
     return true;
 }
 
+// Bookkeeping of a finished launch: every launched scene reports whether its
+// schedule failed, the updates it ran (under a time budget) and its phase
+// timers; requests whose steps are done are retired.  Returns false (with
+// err) when the state readback fails.
+static bool retire(bd_server *S, BatchCore &c, const std::vector<ServedScene *> &launch, uint32_t spl,
+                   std::string &err) {
+    uint32_t lo_slot = UINT32_MAX, hi_slot = 0;
+    for (ServedScene *sc : launch) { lo_slot = std::min(lo_slot, sc->slot); hi_slot = std::max(hi_slot, sc->slot); }
+    std::vector<SceneState> &rb = S->readback;
+    rb.resize(hi_slot - lo_slot + 1);
+    if (hipMemcpyAsync(rb.data(), S->slots.p + lo_slot, rb.size() * sizeof(SceneState), hipMemcpyDeviceToHost,
+                       c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) {
+        err = "scene state readback failed";
+        return false;
+    }
+    const double t_done = S->now();
+    std::vector<bd_request_done> fin;
+    uint64_t pu = 0;
+    for (ServedScene *sc : launch) {
+        Request &rq = sc->queue.front();
+        const SceneState &ss = rb[sc->slot - lo_slot];
+        // whole updates run (under a time budget a scene may stop between
+        // substeps, even before finishing one update: it resumes there)
+        uint32_t ran = std::min(rq.remaining, spl);
+        if (S->budget_ticks && sc->desc.N) ran = std::min(ran, ss.last_steps);
+        rq.remaining -= ran;
+        sc->updates_run += ran;
+        sc->acc_pending = false;
+        sc->last_used = S->launches;
+        sc->heavy_inflight = false;
+        pu += (uint64_t)sc->desc.N * sc->params.sub_steps * ran;
+        // device time per update from the scene's phase timers (a launch that
+        // ended inside an update adds to the next whole one)
+        const uint64_t ticks = ss.t_links + ss.t_build + ss.t_sweep + ss.t_bodies;
+        if (ran) {
+            if (ticks >= sc->ticks_seen) {
+                const double ms = (double)(ticks - sc->ticks_seen) * S->tick_s * 1e3 / ran;
+                sc->ms_per_update = sc->ms_per_update > 0.0 ? 0.5 * (sc->ms_per_update + ms) : ms;
+            }
+            sc->ticks_seen = ticks;
+        }
+        if (ss.err) {
+            // the request ends here with the error; the scene's later
+            // requests are failed before the next launch
+            sc->failed = true;
+            rq.remaining = 0;
+        }
+        if (rq.remaining == 0) {
+            bd_request_done d{};
+            d.request_id = rq.id; d.scene_id = sc->id; d.steps = rq.steps;
+            d.status = sc->failed ? BD_ERR_COLLISION : BD_OK;
+            d.t_submit = rq.t_submit; d.t_start = rq.t_start; d.t_done = t_done;
+            d.particle_updates = sc->failed ? 0 : (uint64_t)sc->desc.N * sc->params.sub_steps * rq.steps;
+            fin.push_back(d);
+            sc->queue.pop_front();
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->launches++;
+        S->pu += pu;
+        S->busy_s += c.last_ms * 1e-3;
+        for (auto &d : fin) S->done.push_back(d);
+        S->n_done += fin.size();
+        S->pending -= fin.size();
+    }
+    if (!fin.empty()) S->idle_cv.notify_all();
+    return true;
+}
+
+// Descriptors of the launch's scenes (each at most spl updates of its oldest
+// request) into core c.
+static void describe_launch(bd_server *S, BatchCore &c, const std::vector<ServedScene *> &launch, uint32_t spl,
+                            double t_start) {
+    c.descs.resize(launch.size());
+    for (size_t i = 0; i < launch.size(); i++) {
+        ServedScene *sc = launch[i];
+        Request &rq = sc->queue.front();
+        if (rq.t_start < 0) rq.t_start = t_start;
+        StepParams sp = sc->params;
+        float mult = 1.0f / (float)sp.sub_steps;  // src/solver.rs:107-108
+        sp.delta = rq.dt * mult;
+        SceneDesc d = sc->desc;
+        d.base = (uint64_t)(uintptr_t)(S->arena.base + sc->off);
+        d.state_slot = sc->slot;
+        d.budget = S->budget_ticks;
+        set_scene_params(d, sp, std::min(rq.remaining, spl), sc->acc_pending);
+        c.descs[i] = d;
+    }
+    std::vector<uint32_t> all(launch.size());
+    std::iota(all.begin(), all.end(), 0u);
+    c.build_order(all);
+    c.descs_dirty = true;
+}
+
 static void scheduler(bd_server *S) {
     std::string err;
     const uint32_t spl = std::max<uint32_t>(S->cfg.steps_per_launch, 1);
@@ -211,9 +323,21 @@ static void scheduler(bd_server *S) {
             }
             // scenes with a pending request, oldest request first
             for (auto &kv : S->scenes)
-                if (!kv.second->queue.empty()) run.push_back(kv.second.get());
+                if (!kv.second->queue.empty() && !kv.second->heavy_inflight) run.push_back(kv.second.get());
         }
-        if (run.empty()) continue;
+        // a finished heavy launch is retired first
+        if (S->heavy_busy && hipStreamQuery(S->heavy.stream) != hipErrorNotReady) {
+            std::lock_guard<std::mutex> lg(S->launch_mu);
+            int rc = S->heavy.finish_timing(err);
+            if (rc != BD_OK || !retire(S, S->heavy, S->heavy_launch, spl, err)) { S->fatal = err; return; }
+            S->heavy_busy = false;
+            S->heavy_launch.clear();
+            continue;  // its scenes may have further requests
+        }
+        if (run.empty()) {
+            if (S->heavy_busy) std::this_thread::sleep_for(std::chrono::microseconds(200));
+            continue;
+        }
         // a scene whose schedule failed runs no more: its requests finish with
         // the error (the kernel would skip its circle pairs, so results would
         // no longer be the reference's)
@@ -250,90 +374,42 @@ static void scheduler(bd_server *S) {
         });
         if (S->cfg.max_active && run.size() > S->cfg.max_active) run.resize(S->cfg.max_active);
         std::lock_guard<std::mutex> lg(S->launch_mu);
-        std::vector<ServedScene *> launch;
+        std::vector<ServedScene *> launch, heavy;
         uint64_t waiting = 0;
         for (ServedScene *sc : run) {
-            if (admit(S, *sc, run, err)) launch.push_back(sc);
+            // heavy scenes wait for the heavy stream to be free
+            const bool is_heavy = sc->ms_per_update > S->heavy_ms;
+            if (is_heavy && S->heavy_busy) continue;
+            if (admit(S, *sc, run, err)) (is_heavy ? heavy : launch).push_back(sc);
             else if (!err.empty()) { S->fatal = err; return; }
             else waiting++;
         }
         S->waiting = waiting;
-        if (launch.empty()) continue;
-        // compact descriptor table for this launch
         const double t_start = S->now();
-        BatchCore &c = S->core;
-        c.descs.resize(launch.size());
-        for (size_t i = 0; i < launch.size(); i++) {
-            ServedScene *sc = launch[i];
-            Request &rq = sc->queue.front();
-            if (rq.t_start < 0) rq.t_start = t_start;
-            StepParams sp = sc->params;
-            float mult = 1.0f / (float)sp.sub_steps;  // src/solver.rs:107-108
-            sp.delta = rq.dt * mult;
-            SceneDesc d = sc->desc;
-            d.base = (uint64_t)(uintptr_t)(S->arena.base + sc->off);
-            d.state_slot = sc->slot;
-            d.budget = S->budget_ticks;
-            set_scene_params(d, sp, std::min(rq.remaining, spl), sc->acc_pending);
-            c.descs[i] = d;
+        if (!heavy.empty()) {
+            // started, not waited for: retired by a later pass of this loop
+            describe_launch(S, S->heavy, heavy, spl, t_start);
+            int rc = S->heavy.run(1, spl, err);
+            if (rc != BD_OK) { S->fatal = err; return; }
+            for (ServedScene *sc : heavy) sc->heavy_inflight = true;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->heavy_launches++;
+            }
+            S->heavy_launch = heavy;
+            S->heavy_busy = true;
         }
-        std::vector<uint32_t> all(launch.size());
-        std::iota(all.begin(), all.end(), 0u);
-        c.build_order(all);
-        c.descs_dirty = true;
+        if (launch.empty()) {
+            // nothing to run now (heavy scenes waiting for their stream, or
+            // scenes waiting for memory): do not spin
+            if (heavy.empty()) std::this_thread::sleep_for(std::chrono::microseconds(200));
+            continue;
+        }
+        BatchCore &c = S->core;
+        describe_launch(S, c, launch, spl, t_start);
         int rc = c.run(1, spl, err);
         if (rc == BD_OK) rc = c.finish_timing(err);
-        if (rc != BD_OK) { S->fatal = err; return; }
-        // every launched scene reports whether its schedule failed and, with a
-        // time budget, the updates it ran
-        uint32_t lo_slot = UINT32_MAX, hi_slot = 0;
-        for (ServedScene *sc : launch) { lo_slot = std::min(lo_slot, sc->slot); hi_slot = std::max(hi_slot, sc->slot); }
-        S->readback.resize(hi_slot - lo_slot + 1);
-        if (hipMemcpyAsync(S->readback.data(), S->slots.p + lo_slot, S->readback.size() * sizeof(SceneState),
-                           hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
-            hipStreamSynchronize(c.stream) != hipSuccess) {
-            S->fatal = "scene state readback failed";
-            return;
-        }
-        const double t_done = S->now();
-        std::vector<bd_request_done> fin;
-        uint64_t pu = 0;
-        for (ServedScene *sc : launch) {
-            Request &rq = sc->queue.front();
-            const SceneState &ss = S->readback[sc->slot - lo_slot];
-            uint32_t ran = std::min(rq.remaining, spl);
-            if (S->budget_ticks && sc->desc.N) ran = std::max(1u, std::min(ran, ss.last_steps));
-            rq.remaining -= ran;
-            sc->updates_run += ran;
-            sc->acc_pending = false;
-            sc->last_used = S->launches;
-            pu += (uint64_t)sc->desc.N * sc->params.sub_steps * ran;
-            if (ss.err) {
-                // the request ends here with the error; the scene's later
-                // requests are failed before the next launch
-                sc->failed = true;
-                rq.remaining = 0;
-            }
-            if (rq.remaining == 0) {
-                bd_request_done d{};
-                d.request_id = rq.id; d.scene_id = sc->id; d.steps = rq.steps;
-                d.status = sc->failed ? BD_ERR_COLLISION : BD_OK;
-                d.t_submit = rq.t_submit; d.t_start = rq.t_start; d.t_done = t_done;
-                d.particle_updates = sc->failed ? 0 : (uint64_t)sc->desc.N * sc->params.sub_steps * rq.steps;
-                fin.push_back(d);
-                sc->queue.pop_front();
-            }
-        }
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->launches++;
-            S->pu += pu;
-            S->busy_s += c.last_ms * 1e-3;
-            for (auto &d : fin) S->done.push_back(d);
-            S->n_done += fin.size();
-            S->pending -= fin.size();
-        }
-        if (!fin.empty()) S->idle_cv.notify_all();
+        if (rc != BD_OK || !retire(S, c, launch, spl, err)) { S->fatal = err; return; }
     }
 }
 
@@ -355,16 +431,23 @@ bd_server *bd_server_new(const bd_server_config *cfg) {
     }
     std::string err;
     if (S->core.init_device(err)) { set_last_error(err); return nullptr; }
+    S->heavy.cfg = S->core.cfg;
+    if (S->heavy.init_device(err)) { set_last_error(err); return nullptr; }
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
+        set_last_error("device wall clock rate unavailable");
+        return nullptr;
+    }
+    S->tick_s = 1.0 / (1000.0 * (double)khz);
     if (cfg->launch_budget_us) {
-        int dev = 0, khz = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) {
-            set_last_error("device wall clock rate unavailable");
-            return nullptr;
-        }
         const uint64_t ticks = (uint64_t)cfg->launch_budget_us * (uint64_t)khz / 1000u;
         S->budget_ticks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ticks, 1), UINT32_MAX);
     }
+    // heavy: ten launch budgets of device time per update (at least 100 ms)
+    // unless the caller says
+    S->heavy_ms = cfg->heavy_update_us ? cfg->heavy_update_us * 1e-3
+                                       : std::max(100.0, 10.0 * cfg->launch_budget_us * 1e-3);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { set_last_error("hipMemGetInfo failed"); return nullptr; }
     uint64_t budget = std::min<uint64_t>(cfg->memory_budget, (uint64_t)(free_b * 0.9));
@@ -374,6 +457,7 @@ bd_server *bd_server_new(const bd_server_config *cfg) {
     S->arena.free_[0] = budget;
     if (S->slots.alloc(kSlotCap) != hipSuccess) { set_last_error("state slots allocation failed"); return nullptr; }
     S->core.states_ptr = S->slots.p;
+    S->heavy.states_ptr = S->slots.p;
     S->free_slots.reserve(kSlotCap);
     for (uint32_t i = kSlotCap; i-- > 0;) S->free_slots.push_back(i);
     S->t0 = std::chrono::steady_clock::now();
@@ -388,6 +472,7 @@ void bd_server_free(bd_server *S) {
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
     if (S->core.stream) (void)hipStreamSynchronize(S->core.stream);
+    if (S->heavy.stream) (void)hipStreamSynchronize(S->heavy.stream);
     if (S->arena.base) (void)hipFree(S->arena.base);
     delete S;
 }
@@ -466,13 +551,19 @@ int32_t bd_server_wait_idle(bd_server *S, double timeout_s) {
 
 int32_t bd_server_read_scene(bd_server *S, int64_t scene_id, bd_solver *out) {
     if (!S || !out) return BD_ERR_ARG;
-    std::lock_guard<std::mutex> lg(S->launch_mu);  // no launch in flight
     ServedScene *sc;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         auto it = S->scenes.find(scene_id);
         if (it == S->scenes.end()) return fail(BD_ERR_INDEX, "unknown scene id");
         sc = it->second.get();
+    }
+    std::unique_lock<std::mutex> lg(S->launch_mu);  // no ordinary launch in flight
+    // nor a heavy launch holding this scene: the scheduler retires it
+    while (sc->heavy_inflight && S->fatal.empty()) {
+        lg.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(500));
+        lg.lock();
     }
     const SceneDesc &d = sc->desc;
     std::vector<uint8_t> buf(sc->init_bytes);
@@ -510,6 +601,7 @@ int32_t bd_server_get_stats(bd_server *S, bd_server_stats *out) {
     out->readmissions = S->readmissions;
     out->waiting_admission = S->waiting;
     out->busy_s = S->busy_s;
+    out->heavy_launches = S->heavy_launches;
     return BD_OK;
 }
 

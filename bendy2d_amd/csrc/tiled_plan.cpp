@@ -84,7 +84,7 @@ bool build_pass(const Plan &p, const TilePlan &tp, const Incidence &inc, const s
                 const uint32_t o = ia[k] == q ? ib[k] : ia[k];
                 if (loc[o] < 0) { loc[o] = (int32_t)R.size(); R.push_back(o); }
             }
-            if (R.size() > region_cap) { ok = false; break; }
+            if (((R.size() + 15) & ~size_t(15)) > region_cap) { ok = false; break; }  // slots fill whole blocks of 16
         }
         if (ok) {
             TileInfo ti{};
@@ -99,19 +99,31 @@ bool build_pass(const Plan &p, const TilePlan &tp, const Incidence &inc, const s
                 out.loff.push_back(cnt);
                 std::vector<uint32_t> &v = per_layer[d];
                 std::sort(v.begin(), v.end());
-                for (uint32_t k : v) {
-                    const Link12 &l = p.links[k];
-                    LinkS r;
-                    r.a = (uint16_t)((uint32_t)loc[ia[k]] | ((l.a & kPinBit) ? kScenePinBit : 0u));
-                    r.b = (uint16_t)((uint32_t)loc[ib[k]] | ((l.b & kPinBit) ? kScenePinBit : 0u));
-                    r.t = l.t;
-                    out.links.push_back(r);
+                // The links of one layer share no point, so their order is
+                // free: deal them out by the bank of their first end (slot
+                // mod 16, one 8-byte slot spans two banks), so that
+                // neighbouring lanes load from different banks.
+                std::vector<uint32_t> by_bank[16];
+                for (uint32_t k : v) by_bank[region_slot((uint32_t)loc[ia[k]]) & 15u].push_back(k);
+                size_t at[16] = {};
+                for (size_t left = v.size(); left;) {
+                    for (uint32_t q = 0; q < 16; q++) {
+                        if (at[q] == by_bank[q].size()) continue;
+                        const uint32_t k = by_bank[q][at[q]++];
+                        left--;
+                        const Link12 &l = p.links[k];
+                        LinkS r;
+                        r.a = (uint16_t)(region_slot((uint32_t)loc[ia[k]]) | ((l.a & kPinBit) ? kScenePinBit : 0u));
+                        r.b = (uint16_t)(region_slot((uint32_t)loc[ib[k]]) | ((l.b & kPinBit) ? kScenePinBit : 0u));
+                        r.t = l.t;
+                        out.links.push_back(r);
+                    }
                 }
                 cnt += (uint32_t)v.size();
             }
             out.loff.push_back(cnt);
             out.link_count += cnt;
-            out.max_region = std::max<uint32_t>(out.max_region, (uint32_t)R.size());
+            out.max_region = std::max<uint32_t>(out.max_region, (uint32_t)((R.size() + 15) & ~size_t(15)));
             out.tiles.push_back(ti);
         }
         for (uint32_t q : R) loc[q] = -1;
@@ -195,16 +207,17 @@ uint64_t tile_plan_selfcheck(const HostScene &s, const Plan &p, const TilePlan &
     for (const TilePass &ps : tp.passes) {
         const uint32_t nl = ps.L1 - ps.L0;
         for (const TileInfo &ti : ps.tiles) {
-            reg.resize(ti.int_count + ti.halo_count);
-            for (uint32_t r = 0; r < ti.int_count; r++) reg[r] = cur[ti.int_start + r];
-            for (uint32_t h = 0; h < ti.halo_count; h++) reg[ti.int_count + h] = cur[ps.halo[ti.halo_off + h]];
+            reg.assign((ti.int_count + ti.halo_count + 15) & ~15u, V2{0.0f, 0.0f});
+            for (uint32_t r = 0; r < ti.int_count; r++) reg[region_slot(r)] = cur[ti.int_start + r];
+            for (uint32_t h = 0; h < ti.halo_count; h++)
+                reg[region_slot(ti.int_count + h)] = cur[ps.halo[ti.halo_off + h]];
             for (uint32_t d = 0; d < nl; d++)
                 for (uint32_t k = ps.loff[ti.loff_off + d]; k < ps.loff[ti.loff_off + d + 1]; k++) {
                     const LinkS &l = ps.links[ti.link_off + k];
                     V2 &A = reg[l.a & 0x7fffu], &B = reg[l.b & 0x7fffu];
                     solve_particle_link(A.x, A.y, B.x, B.y, l.t, (l.a & kScenePinBit) != 0, (l.b & kScenePinBit) != 0);
                 }
-            for (uint32_t r = 0; r < ti.int_count; r++) nxt[ti.int_start + r] = reg[r];
+            for (uint32_t r = 0; r < ti.int_count; r++) nxt[ti.int_start + r] = reg[region_slot(r)];
         }
         std::swap(cur, nxt);
     }

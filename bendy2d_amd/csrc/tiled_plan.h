@@ -25,6 +25,13 @@
 
 namespace bd {
 
+// LDS slot of a tile region's point m: the low four bits of m are XORed
+// with the next four, so that the points of one layer's links, which often
+// share low Morton bits (all left ends of horizontal links have an even x),
+// spread over the 32 four-byte banks.  It permutes each aligned block of 16,
+// so a region of n points needs ceil(n / 16) * 16 slots.
+BD_HD uint32_t region_slot(uint32_t m) { return m ^ ((m >> 4) & 15u); }
+
 struct TileInfo {
     uint32_t int_start, int_count;   // interior: internal points [int_start, int_start + int_count)
     uint32_t halo_off, halo_count;   // halo: internal indices halo[halo_off ...]
@@ -37,7 +44,7 @@ struct TilePass {
     std::vector<TileInfo> tiles;
     std::vector<uint32_t> halo;
     std::vector<uint32_t> loff;      // per tile: L1 - L0 + 1 offsets (relative to the tile's link_off)
-    std::vector<LinkS> links;        // local indices: interior first, then halo
+    std::vector<LinkS> links;        // region slots (region_slot of interior-then-halo indices)
     uint32_t max_region = 0;
     uint64_t link_count = 0;         // links applied per pass (with halo redundancy)
 };

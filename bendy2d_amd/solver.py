@@ -380,27 +380,32 @@ class Batch:
         u = [ctypes.c_uint32() for _ in range(3)]
         f = [ctypes.c_float() for _ in range(2)]
         d = [ctypes.c_double() for _ in range(3)]
-        k = [ctypes.c_uint32(), ctypes.c_uint32()]
+        k = [ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()]
         check(lib().bd_batch_scene_lists(self._h, int(i), *[ctypes.byref(x) for x in u + f + d + k]),
               "batch_scene_lists")
-        names = ("n_out", "n_far", "max_list", "s_cell", "smax", "t_count", "t_lists", "t_sort", "kept", "unchecked")
+        names = ("n_out", "n_far", "max_list", "s_cell", "smax", "t_count", "t_lists", "t_sort", "kept", "unchecked",
+                 "fallbacks")
         return dict(zip(names, [x.value for x in u + f + d + k]))
 
     def cc_stats(self):
-        v = [ctypes.c_uint64() for _ in range(4)]
-        check(lib().bd_batch_cc_stats(self._h, *[ctypes.byref(x) for x in v]), "batch_cc_stats")
-        return dict(zip(("rounds", "redos", "rebuilds", "err_scenes"), [x.value for x in v]))
+        v = [ctypes.c_uint64() for _ in range(6)]
+        check(lib().bd_batch_cc_stats(self._h, *[ctypes.byref(x) for x in v[:4]]), "batch_cc_stats")
+        check(lib().bd_batch_cc_fallbacks(self._h, *[ctypes.byref(x) for x in v[4:]]), "batch_cc_fallbacks")
+        return dict(zip(("rounds", "redos", "rebuilds", "err_scenes", "fallbacks", "fallback_scenes"),
+                        [x.value for x in v]))
 
 
 class Server:
     """Continuous-batching server of scene-advance requests (bd_server_*)."""
 
-    def __init__(self, memory_budget=64 << 30, steps_per_launch=8, max_active=0, launch=None, launch_budget_us=0):
+    def __init__(self, memory_budget=64 << 30, steps_per_launch=8, max_active=0, launch=None, launch_budget_us=0,
+                 heavy_update_us=0):
         cfg = ServerConfig()
         cfg.memory_budget = int(memory_budget)
         cfg.steps_per_launch = int(steps_per_launch)
         cfg.max_active = int(max_active)
         cfg.launch_budget_us = int(launch_budget_us)
+        cfg.heavy_update_us = int(heavy_update_us)
         if launch:
             for k, v in launch.items():
                 setattr(cfg.launch, k, int(v))

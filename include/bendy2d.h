@@ -339,9 +339,16 @@ int32_t bd_batch_last_timing(const bd_batch *b, float *ms, uint32_t *launches);
  * whose schedule failed (their results are not valid). */
 int32_t bd_batch_cc_stats(bd_batch *b, uint64_t *rounds, uint64_t *redos, uint64_t *rebuilds,
                           uint64_t *err_scenes);
+/* Sweeps run by the sequential fallback (a sweep whose candidate lists
+ * failed 8 attempts, or did not fit the workspace at the smallest skin,
+ * runs the reference's (i, j) loop row by row instead), summed over scenes,
+ * and the scenes that ran at least one. */
+int32_t bd_batch_cc_fallbacks(bd_batch *b, uint64_t *fallbacks, uint64_t *scenes);
 /* Circle-pair schedule state of one scene: base skin, error kind (0 none,
- * 1 candidate capacity, 2 no valid sweep in 21 attempts, 3 stalled
- * schedule), rounds, redos, rebuilds and candidate pairs. */
+ * 3 stalled schedule; the kinds 1 and 2 of earlier versions, lists beyond
+ * the workspace and no valid sweep in the attempts allowed, now run the
+ * sequential fallback instead), rounds, redos, rebuilds and candidate
+ * pairs. */
 int32_t bd_batch_scene_status(bd_batch *b, int64_t id, float *skin, uint32_t *err, uint32_t *rounds,
                               uint32_t *redos, uint32_t *rebuilds, uint32_t *pairs);
 /* Device time a scene spent per phase so far, in seconds: link layers,
@@ -354,10 +361,10 @@ int32_t bd_batch_scene_times(bd_batch *b, int64_t id, double *links, double *bui
  * seconds; the sweeps kept although circles moved beyond their skins
  * (every pair missing from the lists was checked not to touch), and those
  * redone because too many circles, or some too far, left their skins to
- * check. */
+ * check, and the sweeps run by the sequential fallback. */
 int32_t bd_batch_scene_lists(bd_batch *b, int64_t id, uint32_t *n_out, uint32_t *n_far, uint32_t *max_list,
                              float *s_cell, float *smax, double *t_count, double *t_lists, double *t_sort,
-                             uint32_t *kept, uint32_t *unchecked);
+                             uint32_t *kept, uint32_t *unchecked, uint32_t *fallbacks);
 
 /* ------------------------------------------------------------------ */
 /* Serving: continuous batching of scene-advance requests              */
@@ -381,9 +388,11 @@ typedef struct bd_server_config {
     uint32_t steps_per_launch;  /* updates per launch for each active scene */
     uint32_t max_active;        /* scenes per launch at most (0 = no limit) */
     bd_batch_config launch;     /* launch shape (wg sizes, split, LDS); steps_per_launch and use_graph ignored */
-    uint32_t launch_budget_us;  /* a scene stops after the update during which it passed this much device
-                                   time in a launch, so slow scenes do not hold up the rest (0: no limit) */
-    uint32_t reserved;          /* zero */
+    uint32_t launch_budget_us;  /* a scene stops after the substep during which it passed this much device
+                                   time in a launch, so slow scenes do not hold up the rest (0: no limit);
+                                   it resumes there in its next launch */
+    uint32_t heavy_update_us;   /* a scene whose updates take more device time than this runs in launches
+                                   of its own on a second stream (0: ten launch budgets, at least 100 ms) */
 } bd_server_config;
 
 /* A finished request.  Times are seconds on the server's monotonic clock
@@ -401,6 +410,7 @@ typedef struct bd_server_stats {
     uint64_t launches, particle_updates, requests_done, requests_pending;
     uint64_t evictions, readmissions, waiting_admission;
     double busy_s;              /* device time of all launches */
+    uint64_t heavy_launches;    /* launches of heavy scenes on the second stream (of `launches`) */
 } bd_server_stats;
 
 bd_server *bd_server_new(const bd_server_config *cfg);
@@ -461,7 +471,7 @@ int32_t bd_scene_image_read(bd_solver *s, const void *host_image, uint64_t bytes
  * reference: for callers that cannot allocate device memory at all (code
  * under graph capture, serving loops, harnesses that forbid hipMalloc).  The
  * workspace is the library's static per-device array of 8 MiB, enough for
- * every scene an image holds (16,384 circles need 6.7 MB); larger
+ * every scene an image holds (16,384 circles need 6.4 MiB); larger
  * workspace_bytes are refused with BD_ERR_CAPACITY.  It allocates nothing.
  * Every call on one device shares that device's array, so calls that overlap
  * on one device (concurrent streams or threads) race; calls in sequence on

@@ -138,14 +138,39 @@ def test_alone_equals_in_mix():
     assert bits_equal(state(a), state(b))
 
 
-def test_failed_schedule_is_reported():
+def test_heavy_scene_runs_beside_the_others(monkeypatch):
+    # a 12,000-circle gas whose every sweep runs sequentially (tens of ms per
+    # update, above a heavy threshold of 5 ms) among small scenes: after its
+    # first launch it runs in launches of its own on the second stream (the
+    # small scenes' requests then finish first when the two streams sit on
+    # different hardware queues); every result equals the oracle
+    heavy = Solver()
+    workloads.gas_box(heavy, n=12000, radius=0.5, box=160.0, n_static=2, seed=21)
+    heavy.set_sub_steps(1)
+    srv = Server(memory_budget=1 << 30, steps_per_launch=4, launch_budget_us=1000, heavy_update_us=5000)
+    monkeypatch.setenv("BD_SCENE_SEQUENTIAL_SWEEP", "1")
+    r_heavy, sid_heavy = srv.create(heavy, 6, DT)
+    monkeypatch.delenv("BD_SCENE_SEQUENTIAL_SWEEP")
+    light = [scene(700 + i, max_side=10) for i in range(6)]
+    ids = [srv.create(s, 24, DT) for s in light]
+    assert srv.wait_idle(300)
+    done = {d["request_id"]: d for d in srv.poll()}
+    assert all(d["status"] == 0 for d in done.values()) and len(done) == 7
+    assert srv.stats()["heavy_launches"] >= 4
+    for s, (_, sid), k in [(heavy, (r_heavy, sid_heavy), 6)] + [(s, i, 24) for s, i in zip(light, ids)]:
+        want = oracle_state(s, [(k, DT)])
+        srv.read_scene(sid, s)
+        got = state(s)
+        assert bits_equal(got, want), f"{first_diff(got, want)}"
+    srv.close()
+
+
+def test_overfull_lists_run_sequentially():
     # 100 circles of radius 1 stacked within 0.05 of one point: every pair is a
-    # candidate (99 per circle), more than the SCENE engine's list capacity of
-    # 32 per circle, so the scene's schedule fails after its bounded retries.
-    # Its requests must finish with BD_ERR_COLLISION (the kernel stops solving
-    # its circle pairs, so results would not be the reference's), later
-    # requests for it too, while a scene in the same launches is unaffected.
-    from bendy2d_amd import _lib
+    # candidate (99 per circle), more than the workspace's 32 per circle at
+    # any skin, so each sweep of this scene runs the sequential fallback.  Its
+    # requests must still equal the oracle, as must a scene in the same
+    # launches.
     bad = Solver()
     bad.gravity = (0.0, 0.0)
     bad.bounds = ((-1000.0, -1000.0), (2000.0, 2000.0))
@@ -160,11 +185,10 @@ def test_failed_schedule_is_reported():
     r_bad2 = srv.advance(sid_bad, 2, DT)
     assert srv.wait_idle(60)
     done = {d["request_id"]: d for d in srv.poll()}
-    assert done[r_bad]["status"] == _lib.BD_ERR_COLLISION
-    assert done[r_bad2]["status"] == _lib.BD_ERR_COLLISION
-    assert done[r_bad]["particle_updates"] == 0
-    assert done[r_good]["status"] == 0
-    want = oracle_state(good, [(5, DT)])
-    srv.read_scene(sid_good, good)
-    assert bits_equal(state(good), want)
+    assert all(done[r]["status"] == 0 for r in (r_bad, r_bad2, r_good))
+    for s, sid, steps in ((bad, sid_bad, 5), (good, sid_good, 5)):
+        want = oracle_state(s, [(steps, DT)])
+        srv.read_scene(sid, s)
+        got = state(s)
+        assert bits_equal(got, want), f"{first_diff(got, want)}"
     srv.close()
