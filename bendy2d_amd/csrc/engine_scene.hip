@@ -148,12 +148,12 @@ constexpr uint32_t kFallbackStreak = 16;
 // its skin from its list-build position (relative to the scene's mean shift).
 constexpr float kDriftRebuild = 1.0f;
 
-// FbGrid: bucket heads (u32, tmask + 1 of them), per-circle next links
-// (u16) and a candidate buffer of cap entries (u16): in LDS when the scene
-// leaves room, else in the workspace.  A circle's bucket is always that of
-// its current position, so it is not stored.
+// FbGrid: bucket heads (u16, tmask + 1 of them, 0xffff for an empty bucket),
+// per-circle next links (u16) and a candidate buffer of cap entries (u16): in
+// LDS when the scene leaves room, else in the workspace.  A circle's bucket
+// is always that of its current position, so it is not stored.
 struct FbGrid {
-    uint32_t *head;
+    uint16_t *head;
     uint16_t *next, *cand;
     uint32_t tmask, cap;
 };
@@ -287,11 +287,12 @@ __device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, floa
 // the outliers.  An outlier-normal pair is found once, by the outlier, and
 // goes into both lists; an outlier-outlier pair is found once per direction
 // (the test is symmetric) and goes into the first one's list.  MODE 0 counts
-// into cnt, MODE 1 appends at the lists' cursors.  Returns the far outliers.
+// into cnt, MODE 1 appends at the lists' cursors.  stage: WG / 2 entries of
+// LDS scratch.  Returns the far outliers.
 template <int WG, int MODE>
 __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, const float *crad, const uint16_t *outs,
                                     uint32_t n_out, float rmax, float s_cell, double h, double inv_h,
-                                    uint32_t tmask, uint32_t *cursor) {
+                                    uint32_t tmask, uint32_t *cursor, float4 *stage) {
     auto add = [&](uint32_t c, uint32_t o) {
         if (MODE == 0) atomicAdd(&w.cnt[c], 1u);
         else w.cand[atomicAdd(&cursor[c], 1u)] = (uint16_t)o;
@@ -336,14 +337,30 @@ __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, c
             if (ex * ex + ey * ey < lim * lim) { add(c, o); add(o, c); }
         }
     }
-    for (uint32_t q = threadIdx.x; q < n_out * n_out; q += WG) {
-        const uint32_t a = q / n_out, b = q - a * n_out;
-        if (a == b) continue;
-        const uint32_t c = outs[a], o = outs[b];
-        float2 p = w.ref[c], po = w.ref[o];
-        float ex = p.x - po.x, ey = p.y - po.y;
-        float lim = (crad[c] + w.skin[c]) + (crad[o] + w.skin[o]);
-        if (ex * ex + ey * ey < lim * lim) add(c, o);
+    // outlier pairs: the outliers' positions and reaches go to LDS, WG / 2 at
+    // a time; a wavefront takes outlier a, its lanes the staged ones
+    constexpr uint32_t kStage = WG / 2;
+    for (uint32_t t0 = 0; t0 < n_out; t0 += kStage) {
+        const uint32_t tn = min(kStage, n_out - t0);
+        __syncthreads();  // the previous tile is no longer read
+        for (uint32_t k = threadIdx.x; k < tn; k += WG) {
+            const uint32_t o = outs[t0 + k];
+            const float2 po = w.ref[o];
+            stage[k] = make_float4(po.x, po.y, crad[o] + w.skin[o], 0.0f);
+        }
+        __syncthreads();
+        for (uint32_t a = wave; a < n_out; a += WG / 64) {
+            const uint32_t c = outs[a];
+            const float2 p = w.ref[c];
+            const float ec = crad[c] + w.skin[c];
+            for (uint32_t k = lane; k < tn; k += 64) {
+                if (t0 + k == a) continue;
+                const float4 q = stage[k];
+                float ex = p.x - q.x, ey = p.y - q.y;
+                float lim = ec + q.z;
+                if (ex * ex + ey * ey < lim * lim) add(c, outs[t0 + k]);
+            }
+        }
     }
     return n_far;
 }
@@ -529,14 +546,23 @@ struct CCStats {
 // the bucket of the cell its current position is in (a push that moves a
 // circle to another bucket moves it in the grid).  Row i gathers every
 // j > i closer than r_i + r_j + M to circle i (nine lanes walk the nine
-// buckets of the 3x3 cells around it); the wavefront ranks them, and one
-// lane runs the pairs in ascending j.  Only circle i moves within its row
+// buckets of the 3x3 cells around it).  The lanes hold the candidates and
+// their positions in registers (a candidate does not move before its turn);
+// the next pair of the row is the smallest candidate above the last one
+// that overlaps circle i where it is now (a wavefront minimum), since the
+// reference's test is false, and the pair a no-op, for every j in between.
+// One lane runs that pair.  Only circle i moves within its row
 // besides each partner at its own turn, so the gather is complete while
 // circle i stays within M/2 of where it was gathered from; when it moves
-// further the rest of the row is gathered again.  The halved margin leaves
-// room for f32 rounding of the distance tests.
+// further the rest of the row is gathered again, with a margin one cell
+// wider each time (up to two cells, over 5x5 and then 7x7 cells), so a
+// circle pushed far within its row is gathered again only a few times.
+// The halved margin leaves room for f32 rounding of the distance tests.
 //
 constexpr uint32_t kNone16 = 0xffffu;
+
+// minimum over the active lanes of the wavefront (ROCm's device library)
+extern "C" __device__ uint32_t __ockl_wfred_min_u32(uint32_t);
 
 template <int WG>
 __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r,
@@ -545,8 +571,10 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
     const uint32_t lane = threadIdx.x & 63u;
     const float rm = fmaxf(rmax, 1e-3f);
     const float M = 0.5f * rm;
-    const double inv_h = 1.0 / (2.0 * (double)rm + (double)M);
-    uint32_t *const head = g.head;
+    const double hd = 2.0 * (double)rm + (double)M;
+    const double inv_h = 1.0 / hd;
+    const float hstep = 0.999f * (float)hd;
+    uint16_t *const head = g.head;
     uint16_t *const next = g.next, *const cand = g.cand;
     const uint32_t tmask = g.tmask;
     auto radius = [=](uint32_t c) { return uni_r ? runi : crad[c]; };
@@ -556,32 +584,39 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
     };
     auto wave_sync = []() { __threadfence_block(); __builtin_amdgcn_wave_barrier(); };
     auto link = [=](uint32_t c, uint32_t k) {  // c into bucket k (lane 0 alone)
-        const uint32_t h0 = head[k];
-        next[c] = h0 == kNone ? kNone16 : (uint16_t)h0;
-        head[k] = c;
+        next[c] = head[k];
+        head[k] = (uint16_t)c;
     };
     auto unlink = [=](uint32_t c, uint32_t k) {  // c out of bucket k (lane 0 alone; lists are acyclic)
         uint32_t x = head[k];
         if (x == c) {
-            head[k] = next[c] == kNone16 ? kNone : next[c];
+            head[k] = next[c];
         } else {
 #pragma unroll 1
-            for (uint32_t guard = 0; x != kNone && guard < Nc; guard++) {
+            for (uint32_t guard = 0; x != kNone16 && guard < Nc; guard++) {
                 const uint32_t y = next[x];
                 if (y == c) { next[x] = next[c]; break; }
-                x = y == kNone16 ? kNone : y;
+                x = y;
             }
         }
     };
+    // heads are u16, two to a 32-bit word: a circle goes in by compare-and-swap
+    // on its bucket's word (each attempt that fails saw another lane succeed)
+    uint32_t *const head32 = (uint32_t *)head;
 #pragma unroll 1
-    for (uint32_t b = lane; b <= tmask; b += 64) head[b] = kNone;
+    for (uint32_t b = lane; b <= tmask / 2u; b += 64) head32[b] = 0xffffffffu;
     wave_sync();
 #pragma unroll 1
     for (uint32_t c = lane; c < Nc; c += 64) {
         const uint32_t k = bucket(P[Np + c]);
         if (k != kNone) {
-            const uint32_t h0 = atomicExch(&head[k], c);
-            next[c] = h0 == kNone ? kNone16 : (uint16_t)h0;
+            const uint32_t sh = 16u * (k & 1u);
+            uint32_t old = head32[k >> 1], seen;
+            do {
+                seen = old;
+                next[c] = (uint16_t)(seen >> sh);
+                old = atomicCAS(&head32[k >> 1], seen, (seen & ~(0xffffu << sh)) | (c << sh));
+            } while (old != seen);
         }
     }
     wave_sync();
@@ -589,63 +624,75 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
     for (uint32_t i = 0; i + 1 < Nc; i++) {
         if (!finite2(P[Np + i])) continue;  // touches nothing (every comparison with it is false)
         const float ri = radius(i);
-        uint32_t jlast = i;
+        uint32_t jlast = i, level = 0, lcap = 2;
+        // a pass runs a pair or ends the row, except a wider gather that
+        // found too many candidates, which the next pass redoes one cell
+        // narrower (the row's widest from then on)
 #pragma unroll 1
-        for (uint32_t pass = 0; pass < Nc; pass++) {
+        for (uint32_t pass = 0; pass < 2u * Nc + 2u; pass++) {
             const float2 base = P[Np + i];
+            // margin of this gather: M, and one cell more for each gather of
+            // the row that circle i outran; the cells within R of its own
+            // hold every circle within reach (R cells apart is more than R h)
+            const float Ml = M + (float)level * hstep;
+            const int R = (int)level + 1, side = 2 * R + 1;
             if (lane == 0) s_fb[0] = 0u;
             wave_sync();
-            // gather: lanes 0-8 walk the distinct buckets of the 3x3 cells
+            // gather: lane q walks the bucket of cell q of the side x side
+            // block (a bucket two cells share is walked twice; its circles are
+            // then listed twice, which changes no pair's turn)
             const int cx0 = scell(base.x, inv_h), cy0 = scell(base.y, inv_h);
-            const uint32_t kk = cell_hash(cx0 + (int)(lane % 3u) - 1, cy0 + (int)(lane / 3u) - 1) & tmask;
-            bool dup = false;
-#pragma unroll 1
-            for (int l2 = 0; l2 < 8; l2++) {
-                const uint32_t other = (uint32_t)__shfl(kk, l2, 64);  // every lane takes part
-                dup |= (uint32_t)l2 < lane && other == kk;
-            }
-            if (lane < 9u && !dup) {
+            if (lane < (uint32_t)(side * side)) {
+                const uint32_t kk =
+                    cell_hash(cx0 + (int)(lane % (uint32_t)side) - R, cy0 + (int)(lane / (uint32_t)side) - R) & tmask;
                 uint32_t j = head[kk];
 #pragma unroll 1
-                for (uint32_t guard = 0; j != kNone && guard < Nc; guard++) {
+                for (uint32_t guard = 0; j != kNone16 && guard < Nc; guard++) {
                     const uint32_t nx = next[j];
                     if (j > jlast) {
                         const float2 pj = P[Np + j];
-                        const float dx = base.x - pj.x, dy = base.y - pj.y, lim = (ri + radius(j)) + M;
+                        const float dx = base.x - pj.x, dy = base.y - pj.y, lim = (ri + radius(j)) + Ml;
                         if (dx * dx + dy * dy < lim * lim) {
                             const uint32_t at = atomicAdd(&s_fb[0], 1u);
                             if (at < g.cap) cand[at] = (uint16_t)j;
                         }
                     }
-                    j = nx == kNone16 ? kNone : nx;
+                    j = nx;
                 }
             }
             wave_sync();
             const uint32_t n = s_fb[0];
             const bool fits = n <= g.cap && n <= kMidList;
+            if (!fits && level > 0u) {
+                lcap = level - 1u;
+                level = lcap;
+                continue;
+            }
             if (fits) {
-                sort_by_wave(cand, n);  // ascending j: the reference's order
-                wave_sync();
-                // the lanes test the next 64 candidates against circle i where
-                // it is now; those before the first that overlaps are no-ops
-                // (the reference's test, false), so one lane runs only the
-                // pairs that overlap, in order
+                constexpr int KM = kMidList / 64;
+                uint32_t v[KM];
+                float2 pv[KM];
+                float rv[KM];
+#pragma unroll
+                for (int m = 0; m < KM; m++) {
+                    const uint32_t k = lane + 64u * m;
+                    v[m] = k < n ? (uint32_t)cand[k] : kNone;
+                    pv[m] = make_float2(0.0f, 0.0f);
+                    rv[m] = 0.0f;
+                    if (v[m] != kNone) { pv[m] = P[Np + v[m]]; rv[m] = ri + radius(v[m]); }
+                }
                 uint32_t again = 0u;
 #pragma unroll 1
-                for (uint32_t k0 = 0; k0 < n;) {
+                for (;;) {
                     const float2 a = P[Np + i];
-                    const uint32_t k = k0 + lane;
-                    bool t = false;
-                    if (k < n) {
-                        const uint32_t j = cand[k];
-                        const float2 pj = P[Np + j];
-                        const float dx = a.x - pj.x, dy = a.y - pj.y, rs = ri + radius(j);
-                        t = dx * dx + dy * dy < rs * rs;
+                    uint32_t best = kNone;
+#pragma unroll
+                    for (int m = 0; m < KM; m++) {
+                        const float dx = a.x - pv[m].x, dy = a.y - pv[m].y;
+                        if (v[m] != kNone && v[m] > jlast && dx * dx + dy * dy < rv[m] * rv[m]) best = min(best, v[m]);
                     }
-                    const uint64_t hit = __ballot(t);
-                    if (!hit) { k0 += 64u; continue; }
-                    const uint32_t kf = k0 + (uint32_t)__builtin_ctzll(hit);
-                    const uint32_t j = cand[kf];
+                    const uint32_t j = __ockl_wfred_min_u32(best);
+                    if (j == kNone) break;  // the row is done
                     if (lane == 0) {
                         float2 ai = a;
                         const float2 b0 = P[Np + j];
@@ -661,16 +708,16 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
                                 if (nb != kNone) link(j, nb);
                             }
                             const float gx = ai.x - base.x, gy = ai.y - base.y;
-                            moved = !(gx * gx + gy * gy <= 0.25f * M * M) ? 1u : 0u;
+                            moved = !(gx * gx + gy * gy <= 0.25f * Ml * Ml) ? 1u : 0u;
                         }
                         s_fb[1] = moved;
                     }
                     wave_sync();
                     jlast = j;
-                    k0 = kf + 1u;
                     if (s_fb[1]) { again = 1u; break; }
                 }
                 if (!again) break;  // the row is done
+                level = min(level + 1u, lcap);
                 continue;           // gather the rest of the row again
             }
             wave_sync();
@@ -682,22 +729,22 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
                     uint32_t j;
                     {
                         // too many to rank: the next j above jlast within reach,
-                        // by walking the nine buckets again (lane 0 alone)
+                        // by walking the block's buckets again (lane 0 alone)
                         j = kNone;
 #pragma unroll 1
-                        for (int q = 0; q < 9; q++) {
-                            const uint32_t kq = cell_hash(cx0 + q % 3 - 1, cy0 + q / 3 - 1) & tmask;
+                        for (int q = 0; q < side * side; q++) {
+                            const uint32_t kq = cell_hash(cx0 + q % side - R, cy0 + q / side - R) & tmask;
                             uint32_t x = head[kq];
 #pragma unroll 1
-                            for (uint32_t guard = 0; x != kNone && guard < Nc; guard++) {
+                            for (uint32_t guard = 0; x != kNone16 && guard < Nc; guard++) {
                                 const uint32_t nx = next[x];
                                 if (x > jlast && x < j) {
                                     const float2 px = P[Np + x];
                                     const float dx = base.x - px.x, dy = base.y - px.y;
-                                    const float lim = (ri + radius(x)) + M;
+                                    const float lim = (ri + radius(x)) + Ml;
                                     if (dx * dx + dy * dy < lim * lim) j = x;
                                 }
-                                x = nx == kNone16 ? kNone : nx;
+                                x = nx;
                             }
                         }
                         if (j == kNone) break;
@@ -715,7 +762,7 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
                             if (nb != kNone) link(j, nb);
                         }
                         const float gx = a.x - base.x, gy = a.y - base.y;
-                        if (!(gx * gx + gy * gy <= 0.25f * M * M)) { again = 1u; break; }
+                        if (!(gx * gx + gy * gy <= 0.25f * Ml * Ml)) { again = 1u; break; }
                     }
                 }
                 s_fb[1] = again;
@@ -724,6 +771,7 @@ __device__ __noinline__ void sequential_sweep(float2 *P, uint32_t Np, uint32_t N
             wave_sync();
             jlast = s_fb[2];
             if (!s_fb[1]) break;
+            level = min(level + 1u, lcap);
         }
     }
     wave_sync();
@@ -744,27 +792,28 @@ __device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t 
     if (fb.q_lds)
         for (uint32_t j = tid; j < fb.nq; j += WG) fb.q_global[j] = fb.q_lds[j];
     __syncthreads();
-    // heads 4 B per bucket, next links 2 B per circle, 256 candidates
+    // heads 2 B per bucket, next links 2 B per circle, 256 candidates
     FbGrid &g = sh->grid;
     const uint64_t fixed = 2ull * Nc + 2ull * kMidList;
     uint32_t tb = 0;
-    if (fixed + 4ull * 64 <= fb.lds_bytes) {
+    if (fixed + 2ull * 64 <= fb.lds_bytes) {
         tb = 64;
-        while (tb < 32768u && tb < 2 * Nc && fixed + 4ull * 2 * tb <= fb.lds_bytes) tb <<= 1;
+        while (tb < 32768u && tb < 2 * Nc && fixed + 2ull * 2 * tb <= fb.lds_bytes) tb <<= 1;
     }
     if (tb) {
         uint8_t *at = fb.lds;
-        g.head = (uint32_t *)at; at += 4ull * tb;
+        g.head = (uint16_t *)at; at += 2ull * tb;
         g.next = (uint16_t *)at; at += 2ull * Nc;
         g.cand = (uint16_t *)at;
         g.tmask = tb - 1;
         g.cap = kMidList;
     } else {
-        // workspace: T + 1 heads (start), next links in key, candidates in stamp
-        g.head = w.start;
+        // workspace: 2T heads (in start's T + 1 words), next links in key,
+        // candidates in stamp
+        g.head = (uint16_t *)w.start;
         g.next = (uint16_t *)w.key;
         g.cand = (uint16_t *)w.stamp;
-        g.tmask = T - 1;
+        g.tmask = 2 * T - 1;
         g.cap = 2 * Nc;
     }
     __syncthreads();  // the grid's description, written by every thread alike, before the first wavefront reads it
@@ -919,7 +968,8 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             }
             __syncthreads();
             const uint32_t n_far =
-                gather_outliers<WG, 0>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, nullptr);
+                gather_outliers<WG, 0>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, nullptr,
+                                      (float4 *)s_wl);
             __syncthreads();
             const uint64_t tb1 = wall_clock64();
             for (uint32_t c = tid; c < Nc; c += WG) w.off[c] = w.cnt[c];
@@ -962,7 +1012,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 w.head[c] = w.off[c] + own;
             }
             __syncthreads();
-            gather_outliers<WG, 1>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, w.head);
+            gather_outliers<WG, 1>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, w.head, (float4 *)s_wl);
             __syncthreads();
             const uint64_t tb2 = wall_clock64();
             // ascending partner order (the reference's pair order): short lists by
@@ -1477,7 +1527,7 @@ __global__ void __launch_bounds__(WG) k_scenes(const SceneDesc *__restrict__ des
     __shared__ float s_red[WG / 64];
     __shared__ uint32_t s_u32[WG / 64];
     __shared__ CCShared s_cc;
-    __shared__ uint32_t s_wl[2 * WG];
+    __shared__ __align__(16) uint32_t s_wl[2 * WG];  // also float4 scratch of the list builds
     __shared__ uint32_t s_next;
     for (;;) {
         if (threadIdx.x == 0) s_next = atomicAdd(queue, 1u);
