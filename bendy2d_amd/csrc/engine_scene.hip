@@ -260,18 +260,37 @@ __device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, floa
 #pragma unroll
         for (int j = 0; j < i; j++) dup[i] = dup[i] || kk[j] == kk[i];
     }
-    uint32_t n = 0;
+    // the buckets' record ranges, loaded together, then their records as one
+    // run, kU at a time (a record sits at start + rank in its bucket's range)
+    uint32_t s0[9], pre[10];
+    pre[0] = 0u;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-        if (dup[i]) continue;
-        const uint32_t s0 = w.start[kk[i]], s1 = w.start[kk[i] + 1];
-#pragma unroll 4
-        for (uint32_t s = s0; s < s1; s++) {
-            const float4 r = w.rec[s];
-            const uint32_t o = __float_as_uint(r.w);
-            if (o == c || (o & kOutlierBit)) continue;  // an outlier lists its pairs itself
-            float ex = p.x - r.x, ey = p.y - r.y;
-            float lim = ec + r.z;
+        const uint32_t a = w.start[kk[i]], b = w.start[kk[i] + 1];
+        s0[i] = a;
+        pre[i + 1] = pre[i] + (dup[i] ? 0u : b - a);
+    }
+    const uint32_t total = pre[9];
+    constexpr uint32_t kU = 4;
+    uint32_t n = 0;
+#pragma unroll 1
+    for (uint32_t t0 = 0; t0 < total; t0 += kU) {
+        float4 r[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t t = t0 + u;
+            uint32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < 9; i++)
+                if (t >= pre[i]) s = s0[i] + (t - pre[i]);
+            r[u] = t < total ? w.rec[s] : make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(c));
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t o = __float_as_uint(r[u].w);
+            if (o == c || (o & kOutlierBit)) continue;  // an outlier lists its pairs itself; padding is c
+            float ex = p.x - r[u].x, ey = p.y - r[u].y;
+            float lim = ec + r[u].z;
             if (ex * ex + ey * ey < lim * lim) {
                 if (n < cap) out[n] = (uint16_t)o;
                 n++;
@@ -310,6 +329,7 @@ __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, c
         for (int q = (int)lane; q < side * side; q += 64) {
             const int cx = cx0 + (q % side) - R, cy = cy0 + (q / side) - R;
             const uint32_t kk = cell_hash(cx, cy) & tmask;
+#pragma unroll 4
             for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
                 const float4 r = w.rec[s];
                 const uint32_t o = __float_as_uint(r.w);
@@ -1117,11 +1137,15 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 const uint32_t xo = ho + 1u < w.cnt[o] ? (uint32_t)w.cand[oo + ho + 1u] : kNone;
                 float qi = 0.0f, qj = 0.0f;
                 const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
+                // loaded beside the list state, not after the test: one global
+                // latency less per round for a pair that touches
+                const float pathc = w.path[c], patho = w.path[o];
+                const float2 savec = w.save[c], saveo = w.save[o];
                 if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
                     P[Np + c] = pi;
                     P[Np + o] = pj;
-                    w.path[c] = excursion(w.path[c], pi.x, pi.y, w.save[c].x, w.save[c].y);
-                    w.path[o] = excursion(w.path[o], pj.x, pj.y, w.save[o].x, w.save[o].y);
+                    w.path[c] = excursion(pathc, pi.x, pi.y, savec.x, savec.y);
+                    w.path[o] = excursion(patho, pj.x, pj.y, saveo.x, saveo.y);
                 }
                 w.traj[oc + hc] = pi;
                 w.traj[oo + ho] = pj;
