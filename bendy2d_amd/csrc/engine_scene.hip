@@ -318,10 +318,31 @@ __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, c
     };
     auto reach_of = [&](float ec) { return ((double)ec + (double)rmax + (double)s_cell) / h; };
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    // the outliers' positions and reaches in LDS (stage), WG / 2 at a time
+    constexpr uint32_t kStage = WG / 2;
+    auto stage_tile = [&](uint32_t t0, uint32_t tn) {
+        __syncthreads();  // the previous tile is no longer read
+        for (uint32_t k = threadIdx.x; k < tn; k += WG) {
+            const uint32_t o = outs[t0 + k];
+            const float2 po = w.ref[o];
+            stage[k] = make_float4(po.x, po.y, crad[o] + w.skin[o], 0.0f);
+        }
+        __syncthreads();
+    };
+    const bool one_tile = n_out <= kStage;
+    if (one_tile) stage_tile(0u, n_out);
     for (uint32_t k = wave; k < n_out; k += WG / 64) {
         const uint32_t c = outs[k];
-        const float2 p = w.ref[c];
-        const float ec = crad[c] + w.skin[c];
+        float2 p;
+        float ec;
+        if (one_tile) {
+            const float4 e = stage[k];
+            p = make_float2(e.x, e.y);
+            ec = e.z;
+        } else {
+            p = w.ref[c];
+            ec = crad[c] + w.skin[c];
+        }
         const double reach = reach_of(ec);
         if (!(reach <= 8.0)) continue;
         const int R = (int)ceil(reach), side = 2 * R + 1;
@@ -357,22 +378,22 @@ __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, c
             if (ex * ex + ey * ey < lim * lim) { add(c, o); add(o, c); }
         }
     }
-    // outlier pairs: the outliers' positions and reaches go to LDS, WG / 2 at
-    // a time; a wavefront takes outlier a, its lanes the staged ones
-    constexpr uint32_t kStage = WG / 2;
+    // outlier pairs: a wavefront takes outlier a, its lanes the staged ones
     for (uint32_t t0 = 0; t0 < n_out; t0 += kStage) {
         const uint32_t tn = min(kStage, n_out - t0);
-        __syncthreads();  // the previous tile is no longer read
-        for (uint32_t k = threadIdx.x; k < tn; k += WG) {
-            const uint32_t o = outs[t0 + k];
-            const float2 po = w.ref[o];
-            stage[k] = make_float4(po.x, po.y, crad[o] + w.skin[o], 0.0f);
-        }
-        __syncthreads();
+        if (!one_tile) stage_tile(t0, tn);
         for (uint32_t a = wave; a < n_out; a += WG / 64) {
             const uint32_t c = outs[a];
-            const float2 p = w.ref[c];
-            const float ec = crad[c] + w.skin[c];
+            float2 p;
+            float ec;
+            if (one_tile) {
+                const float4 e = stage[a];
+                p = make_float2(e.x, e.y);
+                ec = e.z;
+            } else {
+                p = w.ref[c];
+                ec = crad[c] + w.skin[c];
+            }
             for (uint32_t k = lane; k < tn; k += 64) {
                 if (t0 + k == a) continue;
                 const float4 q = stage[k];
@@ -855,6 +876,101 @@ __device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t 
     __syncthreads();
 }
 
+// The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
+// the sweep's list builds and the kernel's link and body phases keep their
+// registers.  nxp and stp are u16 per circle, in LDS or in the workspace.
+template <int WG>
+__device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r,
+                                          float runi, CCShared *sh, uint32_t *s_wl, uint16_t *nxp, uint16_t *stp) {
+    const int tid = threadIdx.x;
+    const CCWork &w = sh->work;
+    // worklists: the first WG ready pairs of a round in LDS (s_wl, two
+    // buffers of WG), the rest in HBM at the same index
+    uint32_t *const gwl[2] = {w.wl, w.wl + (Nc / 2 + 1)};
+    auto wl_put = [&](uint32_t b, uint32_t i, uint32_t v) {
+        if (i < (uint32_t)WG) s_wl[b * WG + i] = v; else gwl[b][i] = v;
+    };
+    auto wl_get = [&](uint32_t b, uint32_t i) { return i < (uint32_t)WG ? s_wl[b * WG + i] : gwl[b][i]; };
+    for (uint32_t c = tid; c < Nc; c += WG) {
+        w.save[c] = P[Np + c];
+        w.path[c] = 0.0f;
+        w.head[c] = 0u;
+        stp[c] = (uint16_t)kNone16;
+        nxp[c] = w.cnt[c] ? w.cand[w.off[c]] : (uint16_t)kNone16;
+    }
+    if (tid == 0) { sh->cnt3[0] = 0u; sh->cnt3[1] = 0u; sh->cnt3[2] = 0u; }
+    __syncthreads();
+    for (uint32_t c = tid; c < Nc; c += WG) {
+        const uint32_t o = nxp[c];
+        if (o != kNone16 && c < o && nxp[o] == c) wl_put(0, atomicAdd(&sh->cnt3[0], 1u), c | (o << 16));
+    }
+    __syncthreads();
+    uint32_t n = uni(sh->cnt3[0]);
+    uint32_t r = 0, r16 = 0;
+    for (; n > 0; r++) {
+        const uint32_t cb = r & 1u, nb = cb ^ 1u;
+        // apply the ready pairs and move both heads; a thread keeps its first
+        // pair and the two new next partners for the readiness check
+        uint32_t mc = 0, mo = 0, mnc = kNone16, mno = kNone16;
+        for (uint32_t k = tid; k < n; k += WG) {
+            const uint32_t pr = wl_get(cb, k), c = pr & 0xffffu, o = pr >> 16;
+            float2 pi = P[Np + c], pj = P[Np + o];
+            const uint32_t hc = w.head[c], ho = w.head[o];
+            const uint32_t oc = w.off[c], oo = w.off[o];
+            const uint32_t xc = hc + 1u < w.cnt[c] ? (uint32_t)w.cand[oc + hc + 1u] : kNone16;
+            const uint32_t xo = ho + 1u < w.cnt[o] ? (uint32_t)w.cand[oo + ho + 1u] : kNone16;
+            float qi = 0.0f, qj = 0.0f;
+            const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
+            // loaded beside the list state, not after the test: one global
+            // latency less per round for a pair that touches
+            const float pathc = w.path[c], patho = w.path[o];
+            const float2 savec = w.save[c], saveo = w.save[o];
+            if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
+                P[Np + c] = pi;
+                P[Np + o] = pj;
+                w.path[c] = excursion(pathc, pi.x, pi.y, savec.x, savec.y);
+                w.path[o] = excursion(patho, pj.x, pj.y, saveo.x, saveo.y);
+            }
+            w.traj[oc + hc] = pi;
+            w.traj[oo + ho] = pj;
+            w.head[c] = hc + 1u;
+            w.head[o] = ho + 1u;
+            nxp[c] = (uint16_t)xc;
+            nxp[o] = (uint16_t)xo;
+            stp[c] = (uint16_t)r16;
+            stp[o] = (uint16_t)r16;
+            if (k == (uint32_t)tid) { mc = c; mo = o; mnc = xc; mno = xo; }
+        }
+        if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;  // read at the end of round r-1, two barriers ago
+        __syncthreads();
+        // pairs completed by a moved head
+        for (uint32_t k = tid; k < n; k += WG) {
+            uint32_t c = mc, o = mo, nc = mnc, no = mno;
+            if (k != (uint32_t)tid) {
+                const uint32_t pr = wl_get(cb, k);
+                c = pr & 0xffffu; o = pr >> 16;
+                nc = nxp[c]; no = nxp[o];
+            }
+            for (int e = 0; e < 2; e++) {
+                const uint32_t x = e ? o : c, y = e ? no : nc;
+                if (y == kNone16 || nxp[y] != x) continue;
+                if (stp[y] == r16 && y < x) continue;  // both moved: the smaller one adds it
+                const uint32_t a = x < y ? x : y, b = x < y ? y : x;
+                wl_put(nb, atomicAdd(&sh->cnt3[(r + 1) % 3], 1u), a | (b << 16));
+            }
+        }
+        __syncthreads();
+        n = uni(sh->cnt3[(r + 1) % 3]);
+        if (tid == 0) sh->cnt.rounds++;
+        if (++r16 == 65535u) {
+            // every round number is about to recur: no stamp may match one
+            for (uint32_t c = tid; c < Nc; c += WG) stp[c] = (uint16_t)kNone16;
+            __syncthreads();
+            r16 = 0u;
+        }
+    }
+}
+
 // One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
 // candidate pairs run in rounds; pair (c, o), c < o, is ready when o is the
 // smallest unprocessed candidate of c and c that of o, so every earlier pair
@@ -874,7 +990,7 @@ template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r, float runi,
                                  float rmax, uint32_t T,
                                  const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, uint32_t *s_wl,
-                                 float &sbase, bool &list_valid, CCStats &cs) {
+                                 float &sbase, bool &list_valid, CCStats &cs, uint16_t *rs_lds) {
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
     // a scene whose lists just failed goes on without them for a few sweeps
@@ -1097,88 +1213,15 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 sh->cnt.t_sort += tb3 - tb2;
             }
         }
-        // ---- sweep: worklist rounds.  nxp (in key, free outside list builds):
-        // each circle's next unprocessed partner, kNone when its list is done,
-        // so a pair (x, y) is ready exactly when nxp[x] == y and nxp[y] == x
-        uint32_t *nxp = w.key;
-        // worklists: the first WG ready pairs of a round in LDS (s_wl, two
-        // buffers of WG), the rest in HBM at the same index
-        uint32_t *const gwl[2] = {w.wl, w.wl + (Nc / 2 + 1)};
-        auto wl_put = [&](uint32_t b, uint32_t i, uint32_t v) {
-            if (i < (uint32_t)WG) s_wl[b * WG + i] = v; else gwl[b][i] = v;
-        };
-        auto wl_get = [&](uint32_t b, uint32_t i) { return i < (uint32_t)WG ? s_wl[b * WG + i] : gwl[b][i]; };
-        for (uint32_t c = tid; c < Nc; c += WG) {
-            w.save[c] = P[Np + c];
-            w.path[c] = 0.0f;
-            w.head[c] = 0u;
-            w.stamp[c] = kNone;
-            nxp[c] = w.cnt[c] ? (uint32_t)w.cand[w.off[c]] : kNone;
-        }
-        if (tid == 0) { sh->cnt3[0] = 0u; sh->cnt3[1] = 0u; sh->cnt3[2] = 0u; }
-        __syncthreads();
-        for (uint32_t c = tid; c < Nc; c += WG) {
-            const uint32_t o = nxp[c];
-            if (o != kNone && c < o && nxp[o] == c) wl_put(0, atomicAdd(&sh->cnt3[0], 1u), c | (o << 16));
-        }
-        __syncthreads();
-        uint32_t n = uni(sh->cnt3[0]);
-        for (uint32_t r = 0; n > 0; r++) {
-            const uint32_t cb = r & 1u, nb = cb ^ 1u;
-            // apply the ready pairs and move both heads; a thread keeps its first
-            // pair and the two new next partners for the readiness check
-            uint32_t mc = 0, mo = 0, mnc = kNone, mno = kNone;
-            for (uint32_t k = tid; k < n; k += WG) {
-                const uint32_t pr = wl_get(cb, k), c = pr & 0xffffu, o = pr >> 16;
-                float2 pi = P[Np + c], pj = P[Np + o];
-                const uint32_t hc = w.head[c], ho = w.head[o];
-                const uint32_t oc = w.off[c], oo = w.off[o];
-                const uint32_t xc = hc + 1u < w.cnt[c] ? (uint32_t)w.cand[oc + hc + 1u] : kNone;
-                const uint32_t xo = ho + 1u < w.cnt[o] ? (uint32_t)w.cand[oo + ho + 1u] : kNone;
-                float qi = 0.0f, qj = 0.0f;
-                const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
-                // loaded beside the list state, not after the test: one global
-                // latency less per round for a pair that touches
-                const float pathc = w.path[c], patho = w.path[o];
-                const float2 savec = w.save[c], saveo = w.save[o];
-                if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
-                    P[Np + c] = pi;
-                    P[Np + o] = pj;
-                    w.path[c] = excursion(pathc, pi.x, pi.y, savec.x, savec.y);
-                    w.path[o] = excursion(patho, pj.x, pj.y, saveo.x, saveo.y);
-                }
-                w.traj[oc + hc] = pi;
-                w.traj[oo + ho] = pj;
-                w.head[c] = hc + 1u;
-                w.head[o] = ho + 1u;
-                nxp[c] = xc;
-                nxp[o] = xo;
-                w.stamp[c] = r;
-                w.stamp[o] = r;
-                if (k == (uint32_t)tid) { mc = c; mo = o; mnc = xc; mno = xo; }
-            }
-            if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;  // read at the end of round r-1, two barriers ago
-            __syncthreads();
-            // pairs completed by a moved head
-            for (uint32_t k = tid; k < n; k += WG) {
-                uint32_t c = mc, o = mo, nc = mnc, no = mno;
-                if (k != (uint32_t)tid) {
-                    const uint32_t pr = wl_get(cb, k);
-                    c = pr & 0xffffu; o = pr >> 16;
-                    nc = nxp[c]; no = nxp[o];
-                }
-                for (int e = 0; e < 2; e++) {
-                    const uint32_t x = e ? o : c, y = e ? no : nc;
-                    if (y == kNone || nxp[y] != x) continue;
-                    if (w.stamp[y] == r && y < x) continue;  // both moved: the smaller one adds it
-                    const uint32_t a = x < y ? x : y, b = x < y ? y : x;
-                    wl_put(nb, atomicAdd(&sh->cnt3[(r + 1) % 3], 1u), a | (b << 16));
-                }
-            }
-            __syncthreads();
-            n = uni(sh->cnt3[(r + 1) % 3]);
-            if (tid == 0) sh->cnt.rounds++;
-        }
+        // ---- sweep: worklist rounds.  nxp: each circle's next unprocessed
+        // partner, kNone16 when its list is done, so a pair (x, y) is ready
+        // exactly when nxp[x] == y and nxp[y] == x; stp: the round (counted
+        // modulo 65535, with a clear at each wrap) in which the circle last
+        // moved.  Both u16: in LDS (rs_lds) when the scene leaves 4 bytes a
+        // circle free there after its resident state, else in the workspace
+        // (key and stamp, free outside list builds).
+        if (rs_lds) sweep_rounds<WG>(P, Np, Nc, crad, uni_r, runi, sh, s_wl, rs_lds, rs_lds + Nc);
+        else sweep_rounds<WG>(P, Np, Nc, crad, uni_r, runi, sh, s_wl, (uint16_t *)w.key, (uint16_t *)w.stamp);
         // every list must be exhausted (a schedule that stalls is an error);
         // circles beyond 0.8 of their skin go to out[] (free after list builds)
         // and vb (stamp, free after the rounds) gets each circle's bound v,
@@ -1318,6 +1361,11 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     }
     const bool links_lds = MODE == 2 && 16ull * N + 8ull * d.L <= lds_bytes;
     LinkS *const s_links = (LinkS *)(s_lds + 2 * N);
+    // the circle-pair rounds' per-circle state (4 bytes a circle) in the LDS
+    // left after what stays resident, when it fits
+    const uint64_t rs_at = MODE == 2 ? 16ull * N + (links_lds ? 8ull * d.L : 0ull) : MODE == 1 ? 8ull * N : 0ull;
+    uint16_t *const rs_lds =
+        rs_at + 4ull * Nc <= lds_bytes ? (uint16_t *)s_lds + rs_at / 2 : nullptr;
     if (links_lds)
         for (uint32_t l = tid; l < d.L; l += WG) s_links[l] = links[l];
     const bool pins = (d.flags & kFlagPins) != 0;
@@ -1439,7 +1487,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                     fallback_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, skin, list_valid);
                 else
                     err = circle_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl,
-                                           skin, list_valid, cs);
+                                           skin, list_valid, cs, rs_lds);
                 if (tid == 0) s_cc->cnt.t_sweep += wall_clock64() - t0;
             }
             // ---- polygon pairs (src/solver.rs:178-187): Polygon::solve_polygon
