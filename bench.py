@@ -116,9 +116,18 @@ def bench_cloth(n, steps, warmup):
     sub, iters = 4, 8
     pu = N * sub * steps
     alg = (32.0 * N + 44.0 * L * iters) * sub * steps
-    return {"pu_per_s": pu / wall, "ms_per_step": 1e3 * wall / steps, "alg_GBps": alg / wall / 1e9,
-            "frac": alg / wall / HBM_PEAK, "engine": s.engine_used, "bodies": N, "links": L,
-            "bytes_per_pu": alg / pu}
+    # measured HBM traffic: one k_tiled_pass launch per link sweep, each
+    # moving the profiled bytes (tiles with their halos, link records)
+    per_pass = profiled_json("k_tiled_pass_cloth1024.md", "traffic_bytes_per_pass") if n == 1024 else None
+    traffic = per_pass * iters * sub * steps if per_pass else None
+    return {"pu_per_s": pu / wall, "ms_per_step": 1e3 * wall / steps,
+            "achieved_GBps": traffic / wall / 1e9 if traffic else None,
+            "frac": traffic / wall / HBM_PEAK if traffic else None,
+            "traffic_per_pass": per_pass,
+            "traffic_source": "profiles/k_tiled_pass_cloth1024.md: rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                              "k_tiled_pass launch, scaled by a 1 GiB copy kernel",
+            "alg_GBps": alg / wall / 1e9, "algorithmic_frac": alg / wall / HBM_PEAK, "engine": s.engine_used,
+            "bodies": N, "links": L, "bytes_per_pu": alg / pu}
 
 
 def bench_gas(steps, warmup):
@@ -447,17 +456,22 @@ def cpu_baselines():
     return data
 
 
-def profiled_traffic_per_update():
-    """HBM bytes per update of the 4,096-scene batch, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel);
-    None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_scenes_batched_softbody4096.md")
+def profiled_json(name, key):
+    """A figure from the JSON block at the end of a committed rocprofv3
+    summary under profiles/ (FETCH_SIZE / WRITE_SIZE scaled by the copy
+    kernel); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         text = open(path).read()
         block = text.rsplit("```json", 1)[1].split("```", 1)[0]
-        return float(json.loads(block)["traffic_bytes_per_update"])
+        return float(json.loads(block)[key])
     except (OSError, IndexError, KeyError, ValueError):
         return None
+
+
+def profiled_traffic_per_update():
+    """HBM bytes per update of the 4,096-scene batch."""
+    return profiled_json("k_scenes_batched_softbody4096.md", "traffic_bytes_per_update")
 
 
 def main():
