@@ -140,6 +140,9 @@ constexpr uint32_t kMidList = 256;  // longer candidate lists are sorted by the 
 constexpr uint32_t kSlot = 24;      // normal partners kept from the counting pass (more are gathered again)
 static_assert(2 * kSlot <= 8 * kCandCap, "count-pass slots must fit in the trajectory array");
 constexpr uint32_t kNone = 0xffffffffu;
+// sweeps with at least this many candidate pairs park previous positions (or
+// positions) in global memory to hold their round state in LDS
+constexpr uint32_t kParkPairs = 32;
 // Failed attempts at a sweep after which it runs sequentially instead.
 constexpr int kFallbackAttempt = 4;
 // Sweeps after such a fallback that run sequentially without trying lists.
@@ -176,6 +179,7 @@ struct CCShared {
         uint8_t *lds;
         uint32_t lds_bytes, nq;
         float2 *q_lds, *q_global;
+        float2 *p_global;  // positions' global copy when they are in LDS without prev (MODE 1)
     } fb;
     FbGrid grid;             // the fallback's grid (see sequential_sweep)
     uint32_t fb_streak;      // sweeps left that go straight to the fallback
@@ -878,28 +882,83 @@ __device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t 
 
 // The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
 // the sweep's list builds and the kernel's link and body phases keep their
-// registers.  nxp and stp are u16 per circle, in LDS or in the workspace.
-template <int WG>
-__device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r,
-                                          float runi, CCShared *sh, uint32_t *s_wl, uint16_t *nxp, uint16_t *stp) {
+// registers.  Per circle, nxp (u16) is its next unprocessed partner, stp (u16)
+// the round in which it last moved and cur (u32) the entry of that partner
+// in the candidate array (its list offset plus its head).  With these in LDS
+// a round's list state is an LDS read, and the next candidate one global
+// read after it.  They sit in the LDS the scene leaves free (rs_lds), or, for
+// a sweep with enough candidate pairs to pay for two copies, in place of the
+// circles' previous positions (held in LDS, unused by the sweep) or of their
+// positions (the rounds then work on the global copy), parked in global
+// memory meanwhile; else in the workspace (key, stamp, head).  The heads go
+// back to the workspace at the end.
+// The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
+// the sweep's list builds and the kernel's link and body phases keep their
+// registers.  Per circle, nxp (u16) is its next unprocessed partner, stp (u16)
+// the round in which it last moved and cur (u32) the entry of that partner
+// in the candidate array (its list offset plus its head).  With these in LDS
+// a round's list state is an LDS read, and the next candidate one global
+// read after it.  They sit in the LDS the scene leaves free (rs_lds), or, for
+// a sweep with enough candidate pairs to pay for two copies, in place of the
+// circles' previous positions (held in LDS, unused by the sweep) or of their
+// positions (the rounds then work on the global copy), parked in global
+// memory meanwhile; else in the workspace (key, stamp, head).  The heads go
+// back to the workspace at the end.
+// A uniform pointer into global memory, typed as such, with its base read
+// from the first lane: loads through it take a 32-bit offset beside a scalar
+// base instead of a 64-bit flat address.
+#define BD_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ BD_GLOBAL T *to_global(T *p) {
+    const uint64_t a = (uint64_t)p;
+    // each half through uni(), which returns it unsigned: the builtin's int,
+    // widened directly, would carry bit 31 of the low half into the high one
+    const uint64_t lo = uni((uint32_t)a);
+    const uint64_t hi = uni((uint32_t)(a >> 32));
+    return (BD_GLOBAL T *)((hi << 32) | lo);
+}
+
+// The rounds themselves (see sweep_rounds); nxp, stp and cur point into LDS
+// or, typed as global, into the workspace.
+template <int WG, typename U16P, typename U32P>
+__device__ __forceinline__ void rounds_core(float2 *Pc, uint32_t Nc, const float *crad_, bool uni_r, float runi,
+                                            CCShared *sh, uint32_t *s_wl, U16P nxp, U16P stp, U32P cur) {
     const int tid = threadIdx.x;
-    const CCWork &w = sh->work;
+    const CCWork &ww = sh->work;
+    struct {
+        BD_GLOBAL const uint32_t *off, *cnt;
+        BD_GLOBAL const uint16_t *cand;
+        BD_GLOBAL float *path;
+        BD_GLOBAL uint64_t *save, *traj;  // f32x2 as their bits
+        BD_GLOBAL uint32_t *head;
+    } w{to_global((const uint32_t *)ww.off), to_global((const uint32_t *)ww.cnt),
+        to_global((const uint16_t *)ww.cand), to_global(ww.path), to_global((uint64_t *)ww.save),
+        to_global((uint64_t *)ww.traj), to_global(ww.head)};
+    auto bits = [](float2 v) { return (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32); };
+    auto f32x2 = [](uint64_t b) { return make_float2(__uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32))); };
+    BD_GLOBAL const float *const crad = to_global(crad_);
     // worklists: the first WG ready pairs of a round in LDS (s_wl, two
     // buffers of WG), the rest in HBM at the same index
-    uint32_t *const gwl[2] = {w.wl, w.wl + (Nc / 2 + 1)};
+    BD_GLOBAL uint32_t *const gwl[2] = {to_global(ww.wl), to_global(ww.wl) + (Nc / 2 + 1)};
     auto wl_put = [&](uint32_t b, uint32_t i, uint32_t v) {
         if (i < (uint32_t)WG) s_wl[b * WG + i] = v; else gwl[b][i] = v;
     };
     auto wl_get = [&](uint32_t b, uint32_t i) { return i < (uint32_t)WG ? s_wl[b * WG + i] : gwl[b][i]; };
+    // the last candidate entry: a cursor's next entry is read before its
+    // list end is known, and only used below that end
+    const uint32_t cand_last = kCandCap * Nc - 1u;
+#pragma unroll 1
     for (uint32_t c = tid; c < Nc; c += WG) {
-        w.save[c] = P[Np + c];
+        w.save[c] = bits(Pc[c]);
         w.path[c] = 0.0f;
-        w.head[c] = 0u;
+        const uint32_t o0 = w.off[c];
+        cur[c] = o0;
         stp[c] = (uint16_t)kNone16;
-        nxp[c] = w.cnt[c] ? w.cand[w.off[c]] : (uint16_t)kNone16;
+        nxp[c] = w.cnt[c] ? w.cand[o0] : (uint16_t)kNone16;
     }
     if (tid == 0) { sh->cnt3[0] = 0u; sh->cnt3[1] = 0u; sh->cnt3[2] = 0u; }
     __syncthreads();
+#pragma unroll 1
     for (uint32_t c = tid; c < Nc; c += WG) {
         const uint32_t o = nxp[c];
         if (o != kNone16 && c < o && nxp[o] == c) wl_put(0, atomicAdd(&sh->cnt3[0], 1u), c | (o << 16));
@@ -907,34 +966,37 @@ __device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, c
     __syncthreads();
     uint32_t n = uni(sh->cnt3[0]);
     uint32_t r = 0, r16 = 0;
+#pragma unroll 1
     for (; n > 0; r++) {
         const uint32_t cb = r & 1u, nb = cb ^ 1u;
         // apply the ready pairs and move both heads; a thread keeps its first
         // pair and the two new next partners for the readiness check
         uint32_t mc = 0, mo = 0, mnc = kNone16, mno = kNone16;
+#pragma unroll 1
         for (uint32_t k = tid; k < n; k += WG) {
             const uint32_t pr = wl_get(cb, k), c = pr & 0xffffu, o = pr >> 16;
-            float2 pi = P[Np + c], pj = P[Np + o];
-            const uint32_t hc = w.head[c], ho = w.head[o];
-            const uint32_t oc = w.off[c], oo = w.off[o];
-            const uint32_t xc = hc + 1u < w.cnt[c] ? (uint32_t)w.cand[oc + hc + 1u] : kNone16;
-            const uint32_t xo = ho + 1u < w.cnt[o] ? (uint32_t)w.cand[oo + ho + 1u] : kNone16;
+            float2 pi = Pc[c], pj = Pc[o];
+            const uint32_t uc = cur[c], uo = cur[o];
+            const uint32_t ec = w.off[c + 1], eo = w.off[o + 1];  // list ends
+            const uint32_t yc = w.cand[min(uc + 1u, cand_last)], yo = w.cand[min(uo + 1u, cand_last)];
+            const uint32_t xc = uc + 1u < ec ? yc : kNone16;
+            const uint32_t xo = uo + 1u < eo ? yo : kNone16;
             float qi = 0.0f, qj = 0.0f;
             const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
             // loaded beside the list state, not after the test: one global
             // latency less per round for a pair that touches
             const float pathc = w.path[c], patho = w.path[o];
-            const float2 savec = w.save[c], saveo = w.save[o];
+            const float2 savec = f32x2(w.save[c]), saveo = f32x2(w.save[o]);
             if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
-                P[Np + c] = pi;
-                P[Np + o] = pj;
+                Pc[c] = pi;
+                Pc[o] = pj;
                 w.path[c] = excursion(pathc, pi.x, pi.y, savec.x, savec.y);
                 w.path[o] = excursion(patho, pj.x, pj.y, saveo.x, saveo.y);
             }
-            w.traj[oc + hc] = pi;
-            w.traj[oo + ho] = pj;
-            w.head[c] = hc + 1u;
-            w.head[o] = ho + 1u;
+            w.traj[uc] = bits(pi);
+            w.traj[uo] = bits(pj);
+            cur[c] = uc + 1u;
+            cur[o] = uo + 1u;
             nxp[c] = (uint16_t)xc;
             nxp[o] = (uint16_t)xo;
             stp[c] = (uint16_t)r16;
@@ -944,6 +1006,7 @@ __device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, c
         if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;  // read at the end of round r-1, two barriers ago
         __syncthreads();
         // pairs completed by a moved head
+#pragma unroll 1
         for (uint32_t k = tid; k < n; k += WG) {
             uint32_t c = mc, o = mo, nc = mnc, no = mno;
             if (k != (uint32_t)tid) {
@@ -951,6 +1014,7 @@ __device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, c
                 c = pr & 0xffffu; o = pr >> 16;
                 nc = nxp[c]; no = nxp[o];
             }
+#pragma unroll 1
             for (int e = 0; e < 2; e++) {
                 const uint32_t x = e ? o : c, y = e ? no : nc;
                 if (y == kNone16 || nxp[y] != x) continue;
@@ -964,11 +1028,68 @@ __device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, c
         if (tid == 0) sh->cnt.rounds++;
         if (++r16 == 65535u) {
             // every round number is about to recur: no stamp may match one
+#pragma unroll 1
             for (uint32_t c = tid; c < Nc; c += WG) stp[c] = (uint16_t)kNone16;
             __syncthreads();
             r16 = 0u;
         }
     }
+#pragma unroll 1
+    for (uint32_t c = tid; c < Nc; c += WG) w.head[c] = cur[c] - w.off[c];
+}
+
+// The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
+// the sweep's list builds and the kernel's link and body phases keep their
+// registers.  Per circle, nxp (u16) is its next unprocessed partner, stp (u16)
+// the round in which it last moved and cur (u32) the entry of that partner
+// in the candidate array (its list offset plus its head).  With these in LDS
+// a round's list state is an LDS read, and the next candidate one global
+// read after it.  They sit in the LDS the scene leaves free (rs_off bytes
+// into the dynamic LDS, kNone for none), or, for a sweep with enough
+// candidate pairs to pay for two copies, in place of the circles' previous
+// positions (held in LDS, unused by the sweep) or of their positions (the
+// rounds then work on the global copy), parked in global memory meanwhile;
+// else in the workspace (key, stamp, head).  The rounds are instantiated
+// once for each placement, so the LDS one addresses LDS directly.  The heads
+// go back to the workspace at the end.
+template <int WG>
+__device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r,
+                                          float runi, CCShared *sh, uint32_t *s_wl, uint32_t rs_off,
+                                          uint32_t pairs) {
+    extern __shared__ float2 s_dyn[];
+    const int tid = threadIdx.x;
+    const CCWork &w = sh->work;
+    const CCShared::Fallback &fb = sh->fb;
+    float2 *Pc = P + Np, *pl = nullptr, *pg = nullptr;
+    if (rs_off == kNone && pairs >= kParkPairs) {
+        if (fb.q_lds) {
+            pl = fb.q_lds + Np;
+            pg = fb.q_global + Np;
+        } else if (fb.p_global) {
+            pl = P + Np;
+            pg = fb.p_global + Np;
+            Pc = pg;
+        }
+        if (pl) {
+#pragma unroll 1
+            for (uint32_t c = tid; c < Nc; c += WG) pg[c] = pl[c];
+            __syncthreads();
+            rs_off = (uint32_t)((uint8_t *)pl - (uint8_t *)s_dyn);
+        }
+    }
+    if (rs_off != kNone) {
+        uint16_t *const rs = (uint16_t *)((uint8_t *)s_dyn + rs_off);
+        rounds_core<WG>(Pc, Nc, crad, uni_r, runi, sh, s_wl, rs, rs + Nc, (uint32_t *)(rs + 2 * Nc));
+    } else {
+        BD_GLOBAL uint16_t *const key = to_global((uint16_t *)w.key);
+        rounds_core<WG>(Pc, Nc, crad, uni_r, runi, sh, s_wl, key, to_global((uint16_t *)w.stamp),
+                        to_global(w.head));
+    }
+    if (pl) {
+#pragma unroll 1
+        for (uint32_t c = tid; c < Nc; c += WG) pl[c] = pg[c];
+    }
+    __syncthreads();
 }
 
 // One circle-pair sweep of a scene (src/solver.rs:168-177), exact: the
@@ -990,7 +1111,7 @@ template <int WG>
 __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const float *crad, bool uni_r, float runi,
                                  float rmax, uint32_t T,
                                  const CCWork &w, CCShared *sh, float *s_red, uint32_t *s_u32, uint32_t *s_wl,
-                                 float &sbase, bool &list_valid, CCStats &cs, uint16_t *rs_lds) {
+                                 float &sbase, bool &list_valid, CCStats &cs, uint32_t rs_off) {
     const int tid = threadIdx.x;
     const uint32_t tmask = T - 1;
     // a scene whose lists just failed goes on without them for a few sweeps
@@ -1217,11 +1338,8 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         // partner, kNone16 when its list is done, so a pair (x, y) is ready
         // exactly when nxp[x] == y and nxp[y] == x; stp: the round (counted
         // modulo 65535, with a clear at each wrap) in which the circle last
-        // moved.  Both u16: in LDS (rs_lds) when the scene leaves 4 bytes a
-        // circle free there after its resident state, else in the workspace
-        // (key and stamp, free outside list builds).
-        if (rs_lds) sweep_rounds<WG>(P, Np, Nc, crad, uni_r, runi, sh, s_wl, rs_lds, rs_lds + Nc);
-        else sweep_rounds<WG>(P, Np, Nc, crad, uni_r, runi, sh, s_wl, (uint16_t *)w.key, (uint16_t *)w.stamp);
+        // moved (see sweep_rounds).
+        sweep_rounds<WG>(P, Np, Nc, crad, uni_r, runi, sh, s_wl, rs_off, cs.pairs);
         // every list must be exhausted (a schedule that stalls is an error);
         // circles beyond 0.8 of their skin go to out[] (free after list builds)
         // and vb (stamp, free after the rounds) gets each circle's bound v,
@@ -1361,11 +1479,10 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     }
     const bool links_lds = MODE == 2 && 16ull * N + 8ull * d.L <= lds_bytes;
     LinkS *const s_links = (LinkS *)(s_lds + 2 * N);
-    // the circle-pair rounds' per-circle state (4 bytes a circle) in the LDS
+    // the circle-pair rounds' per-circle state (8 bytes a circle) in the LDS
     // left after what stays resident, when it fits
     const uint64_t rs_at = MODE == 2 ? 16ull * N + (links_lds ? 8ull * d.L : 0ull) : MODE == 1 ? 8ull * N : 0ull;
-    uint16_t *const rs_lds =
-        rs_at + 4ull * Nc <= lds_bytes ? (uint16_t *)s_lds + rs_at / 2 : nullptr;
+    const uint32_t rs_off = rs_at + 8ull * Nc <= lds_bytes ? (uint32_t)rs_at : kNone;
     if (links_lds)
         for (uint32_t l = tid; l < d.L; l += WG) s_links[l] = links[l];
     const bool pins = (d.flags & kFlagPins) != 0;
@@ -1373,10 +1490,12 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
         CCShared::Fallback &f = s_cc->fb;
         if constexpr (MODE == 2) {
             f.lds = (uint8_t *)Q; f.lds_bytes = 8u * N; f.q_lds = Q; f.q_global = gprev; f.nq = N;
+            f.p_global = nullptr;
         } else {
             const uint32_t used = MODE == 1 ? 8u * N : 0u;
             f.lds = (uint8_t *)(s_lds) + used; f.lds_bytes = lds_bytes > used ? lds_bytes - used : 0u;
             f.q_lds = nullptr; f.q_global = nullptr; f.nq = 0u;
+            f.p_global = MODE == 1 ? gpos : nullptr;
         }
     }
     __syncthreads();
@@ -1487,7 +1606,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                     fallback_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, skin, list_valid);
                 else
                     err = circle_sweep<WG>(P, Np, Nc, crad, uni_r, runi, d.rmax, d.T, w, s_cc, s_red, s_u32, s_wl,
-                                           skin, list_valid, cs, rs_lds);
+                                           skin, list_valid, cs, rs_off);
                 if (tid == 0) s_cc->cnt.t_sweep += wall_clock64() - t0;
             }
             // ---- polygon pairs (src/solver.rs:178-187): Polygon::solve_polygon
