@@ -50,6 +50,21 @@ struct ColliderStage {
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float uni(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
 
+// A uniform pointer into global memory, typed as such, with its base read
+// from the first lane: loads through it take a 32-bit offset beside a scalar
+// base instead of a 64-bit flat address.
+#define BD_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ BD_GLOBAL T *to_global(T *p) {
+    const uint64_t a = (uint64_t)p;
+    // each half through uni(), which returns it unsigned: the builtin's int,
+    // widened directly, would carry bit 31 of the low half into the high one
+    const uint64_t lo = uni((uint32_t)a);
+    const uint64_t hi = uni((uint32_t)(a >> 32));
+    return (BD_GLOBAL T *)((hi << 32) | lo);
+}
+
+
 template <int WG>
 __device__ __forceinline__ float wg_max(float v, float *s_red) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -904,20 +919,6 @@ __device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t 
 // positions (the rounds then work on the global copy), parked in global
 // memory meanwhile; else in the workspace (key, stamp, head).  The heads go
 // back to the workspace at the end.
-// A uniform pointer into global memory, typed as such, with its base read
-// from the first lane: loads through it take a 32-bit offset beside a scalar
-// base instead of a 64-bit flat address.
-#define BD_GLOBAL __attribute__((address_space(1)))
-template <typename T>
-__device__ __forceinline__ BD_GLOBAL T *to_global(T *p) {
-    const uint64_t a = (uint64_t)p;
-    // each half through uni(), which returns it unsigned: the builtin's int,
-    // widened directly, would carry bit 31 of the low half into the high one
-    const uint64_t lo = uni((uint32_t)a);
-    const uint64_t hi = uni((uint32_t)(a >> 32));
-    return (BD_GLOBAL T *)((hi << 32) | lo);
-}
-
 // The rounds themselves (see sweep_rounds); nxp, stp and cur point into LDS
 // or, typed as global, into the workspace.
 template <int WG, typename U16P, typename U32P>
@@ -1086,6 +1087,8 @@ __device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, c
                         to_global(w.head));
     }
     if (pl) {
+        // every cursor has been read (the state sits where the positions go back)
+        __syncthreads();
 #pragma unroll 1
         for (uint32_t c = tid; c < Nc; c += WG) pl[c] = pg[c];
     }
