@@ -187,3 +187,41 @@ def test_batch_dense_contact_exact():
     assert sum(x["rounds"] for x in st) > 0
     kept = sum(b.scene_lists(i)["kept"] for i in range(len(scenes)))
     print("sweeps kept past skins:", kept, "redos:", sum(x["redos"] for x in st))
+
+
+_placement_oracle = {}
+
+
+@pytest.mark.parametrize("lds_small, placement", [
+    (49152, "LDS beside positions and previous positions"),
+    (24576, "previous positions parked, state in their place"),
+    (16384, "positions parked (positions alone fit), state in their place"),
+    (8192, "positions in global memory, state in the workspace"),
+])
+def test_round_state_placements(lds_small, placement):
+    # the pair rounds keep each circle's next partner, stamp and list cursor
+    # in LDS when the scene's LDS budget leaves 8 bytes a circle, else in
+    # place of the previous positions or positions (parked in global memory
+    # for the sweep), else in the workspace; for a stack of about 1,400
+    # circles in lasting contact these budgets select each case in turn, and
+    # every one must match the oracle bit for bit
+    scenes = [circle_column(12, 120, 3), circle_column(10, 140, 4)]
+    steps = 120
+    if "want" not in _placement_oracle:
+        _placement_oracle["want"] = oracle_run(scenes, steps)
+    want = _placement_oracle["want"]
+    b = Batch()
+    for s in scenes:
+        b.add_scene(s)
+    b.config = {"wg_small": 256, "wg_large": 1024, "split_points": 2048, "lds_small": lds_small,
+                "lds_large": 147456, "steps_per_launch": 2, "use_graph": 0, "split_tiny": 0, "wg_tiny": 64,
+                "lds_tiny": 8192}
+    b.update(DT, steps)
+    b.sync()
+    for i, s in enumerate(scenes):
+        b.read_scene(i, s)
+        a = state(s)
+        assert bits_equal(a, want[i]), f"{placement}: scene {i} first diff {first_diff(a, want[i])}"
+    st = [b.scene_status(i) for i in range(len(scenes))]
+    assert all(x["err"] == 0 for x in st), st
+    assert sum(x["rounds"] for x in st) > 0
