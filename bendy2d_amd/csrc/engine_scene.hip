@@ -64,6 +64,17 @@ __device__ __forceinline__ BD_GLOBAL T *to_global(T *p) {
     return (BD_GLOBAL T *)((hi << 32) | lo);
 }
 
+// The same for read-only data read at uniform indices: typed as constant, it
+// is read by scalar loads into scalar registers.
+#define BD_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ BD_CONST const T *to_const(const T *p) {
+    const uint64_t a = (uint64_t)p;
+    const uint64_t lo = uni((uint32_t)a);
+    const uint64_t hi = uni((uint32_t)(a >> 32));
+    return (BD_CONST const T *)((hi << 32) | lo);
+}
+
 
 template <int WG>
 __device__ __forceinline__ float wg_max(float v, float *s_red) {
@@ -1448,6 +1459,9 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
     const float *crad = (const float *)(base + d.crad_off);
     const uint32_t *pinmask = (const uint32_t *)(base + d.pin_off);
     const uint32_t *loff = (const uint32_t *)(base + d.loff_off);
+    // layer bounds: every lane reads the same two before each layer, through
+    // the scalar cache (a vector load here put HBM latency before every layer)
+    BD_CONST const uint32_t *const kloff = to_const(loff);
     const LinkS *links = (const LinkS *)(base + d.links_off);
     float2 *P, *Q;
     if constexpr (MODE >= 1) P = s_lds; else P = gpos;
@@ -1569,7 +1583,7 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             for (uint32_t it = 0; it < d.link_iters; it++) {
                 if (npoly && it + 1 == d.link_iters) poly_centres();
                 for (uint32_t L = 0; L < d.depth; L++) {
-                    uint32_t lo = loff[L], hi = loff[L + 1];
+                    uint32_t lo = kloff[L], hi = kloff[L + 1];
                     // not unrolled (measured faster than unrolling by 2, 4 or 8,
                     // and than loading the next layer's first record early:
                     // register pressure of the whole persistent kernel); the
