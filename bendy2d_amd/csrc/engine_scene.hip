@@ -75,6 +75,16 @@ __device__ __forceinline__ BD_CONST const T *to_const(const T *p) {
     return (BD_CONST const T *)((hi << 32) | lo);
 }
 
+// A pointer into LDS, typed as such: the low 32 bits of a generic LDS address
+// are its LDS offset.  Accesses through it are LDS instructions, where a
+// generic pointer takes flat ones (slower, and counted against both the LDS
+// and the vector-memory wait counters).
+#define BD_LDS __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ BD_LDS T *to_lds(T *p) {
+    return (BD_LDS T *)(uint32_t)(uint64_t)p;
+}
+
 
 template <int WG>
 __device__ __forceinline__ float wg_max(float v, float *s_red) {
@@ -89,28 +99,33 @@ __device__ __forceinline__ float wg_max(float v, float *s_red) {
     return uni(m);
 }
 
+// inclusive prefix sum over the lanes of the wavefront, all lanes active
+// (ROCm's device library: a row-shift and row-broadcast sequence)
+extern "C" __device__ uint32_t __ockl_wfscan_add_u32(uint32_t, bool);
+
 // Exclusive scan in place over a[0..n) in global memory by one workgroup.
 template <int WG>
 __device__ void wg_scan_global(uint32_t *a, uint32_t n, uint32_t *s_wave) {
     uint32_t carry = 0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // The wave index is uniform and the lane scan takes no lane masks: the
+    // loops over the waves below are scalar loops, and no per-lane compare
+    // stays live for the whole kernel.
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = uni(threadIdx.x >> 6);
     for (uint32_t base = 0; base < n; base += WG * 4) {
         uint32_t i0 = base + threadIdx.x * 4;
         uint32_t v[4], local = 0;
         for (int k = 0; k < 4; k++) { v[k] = (i0 + k < n) ? a[i0 + k] : 0u; local += v[k]; }
-        uint32_t inc = local;
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t t = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += t;
-        }
+        const uint32_t inc = __ockl_wfscan_add_u32(local, true);
         __syncthreads();
         if (lane == 63) s_wave[wave] = inc;
         __syncthreads();
         uint32_t woff = 0, total = 0;
-        for (int w = 0; w < WG / 64; w++) {
-            if (w < wave) woff += s_wave[w];
-            total += s_wave[w];
-        }
+#pragma unroll 1
+        for (uint32_t w = 0; w < wave; w++) woff += s_wave[w];
+        total = woff;
+#pragma unroll 1
+        for (uint32_t w = wave; w < WG / 64; w++) total += s_wave[w];
         uint32_t run = carry + woff + inc - local;
         for (int k = 0; k < 4; k++) {
             if (i0 + k < n) a[i0 + k] = run;
@@ -932,8 +947,8 @@ __device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t 
 // back to the workspace at the end.
 // The rounds themselves (see sweep_rounds); nxp, stp and cur point into LDS
 // or, typed as global, into the workspace.
-template <int WG, typename U16P, typename U32P>
-__device__ __forceinline__ void rounds_core(float2 *Pc, uint32_t Nc, const float *crad_, bool uni_r, float runi,
+template <int WG, typename PCP, typename U16P, typename U32P>
+__device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *crad_, bool uni_r, float runi,
                                             CCShared *sh, uint32_t *s_wl, U16P nxp, U16P stp, U32P cur) {
     const int tid = threadIdx.x;
     const CCWork &ww = sh->work;
@@ -961,7 +976,7 @@ __device__ __forceinline__ void rounds_core(float2 *Pc, uint32_t Nc, const float
     const uint32_t cand_last = kCandCap * Nc - 1u;
 #pragma unroll 1
     for (uint32_t c = tid; c < Nc; c += WG) {
-        w.save[c] = bits(Pc[c]);
+        w.save[c] = Pc[c];
         w.path[c] = 0.0f;
         const uint32_t o0 = w.off[c];
         cur[c] = o0;
@@ -987,7 +1002,7 @@ __device__ __forceinline__ void rounds_core(float2 *Pc, uint32_t Nc, const float
 #pragma unroll 1
         for (uint32_t k = tid; k < n; k += WG) {
             const uint32_t pr = wl_get(cb, k), c = pr & 0xffffu, o = pr >> 16;
-            float2 pi = Pc[c], pj = Pc[o];
+            float2 pi = f32x2(Pc[c]), pj = f32x2(Pc[o]);
             const uint32_t uc = cur[c], uo = cur[o];
             const uint32_t ec = w.off[c + 1], eo = w.off[o + 1];  // list ends
             const uint32_t yc = w.cand[min(uc + 1u, cand_last)], yo = w.cand[min(uo + 1u, cand_last)];
@@ -1000,8 +1015,8 @@ __device__ __forceinline__ void rounds_core(float2 *Pc, uint32_t Nc, const float
             const float pathc = w.path[c], patho = w.path[o];
             const float2 savec = f32x2(w.save[c]), saveo = f32x2(w.save[o]);
             if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
-                Pc[c] = pi;
-                Pc[o] = pj;
+                Pc[c] = bits(pi);
+                Pc[o] = bits(pj);
                 w.path[c] = excursion(pathc, pi.x, pi.y, savec.x, savec.y);
                 w.path[o] = excursion(patho, pj.x, pj.y, saveo.x, saveo.y);
             }
@@ -1089,13 +1104,26 @@ __device__ __noinline__ void sweep_rounds(float2 *P, uint32_t Np, uint32_t Nc, c
             rs_off = (uint32_t)((uint8_t *)pl - (uint8_t *)s_dyn);
         }
     }
+    // the positions as 64-bit words, typed by where they are (LDS, or global
+    // memory for a scene without them in LDS or with them parked): a generic
+    // LDS address carries the LDS aperture in its high half
+    const bool pc_lds = ((uint64_t)Pc >> 32) == ((uint64_t)s_dyn >> 32);
     if (rs_off != kNone) {
         uint16_t *const rs = (uint16_t *)((uint8_t *)s_dyn + rs_off);
-        rounds_core<WG>(Pc, Nc, crad, uni_r, runi, sh, s_wl, rs, rs + Nc, (uint32_t *)(rs + 2 * Nc));
+        if (pc_lds)
+            rounds_core<WG>(to_lds((uint64_t *)Pc), Nc, crad, uni_r, runi, sh, s_wl, rs, rs + Nc,
+                            (uint32_t *)(rs + 2 * Nc));
+        else
+            rounds_core<WG>(to_global((uint64_t *)Pc), Nc, crad, uni_r, runi, sh, s_wl, rs, rs + Nc,
+                            (uint32_t *)(rs + 2 * Nc));
     } else {
         BD_GLOBAL uint16_t *const key = to_global((uint16_t *)w.key);
-        rounds_core<WG>(Pc, Nc, crad, uni_r, runi, sh, s_wl, key, to_global((uint16_t *)w.stamp),
-                        to_global(w.head));
+        BD_GLOBAL uint16_t *const stamp = to_global((uint16_t *)w.stamp);
+        BD_GLOBAL uint32_t *const head = to_global(w.head);
+        if (pc_lds)
+            rounds_core<WG>(to_lds((uint64_t *)Pc), Nc, crad, uni_r, runi, sh, s_wl, key, stamp, head);
+        else
+            rounds_core<WG>(to_global((uint64_t *)Pc), Nc, crad, uni_r, runi, sh, s_wl, key, stamp, head);
     }
     if (pl) {
         // every cursor has been read (the state sits where the positions go back)
