@@ -346,8 +346,10 @@ __device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, floa
 }
 
 // Outliers' partners.  A near outlier (its normal partners lie within 8
-// cells) is searched by one wavefront over those cells, a far one by the whole
-// workgroup over every circle, and pairs of two outliers by all pairs among
+// cells) has those cells scanned: the (outlier, cell) items of all near
+// outliers are spread over the workgroup's threads when the outliers fit one
+// stage, else a wavefront takes one outlier at a time.  A far one is searched
+// by the whole workgroup over every circle, and pairs of two outliers by all pairs among
 // the outliers.  An outlier-normal pair is found once, by the outlier, and
 // goes into both lists; an outlier-outlier pair is found once per direction
 // (the test is symmetric) and goes into the first one's list.  MODE 0 counts
@@ -356,7 +358,7 @@ __device__ __noinline__ uint32_t gather_normal(uint32_t c, const CCWork &w, floa
 template <int WG, int MODE>
 __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, const float *crad, const uint16_t *outs,
                                     uint32_t n_out, float rmax, float s_cell, double h, double inv_h,
-                                    uint32_t tmask, uint32_t *cursor, float4 *stage) {
+                                    uint32_t tmask, uint32_t *cursor, float4 *stage, uint32_t *s_wave) {
     auto add = [&](uint32_t c, uint32_t o) {
         if (MODE == 0) atomicAdd(&w.cnt[c], 1u);
         else w.cand[atomicAdd(&cursor[c], 1u)] = (uint16_t)o;
@@ -375,37 +377,94 @@ __device__ __noinline__ uint32_t gather_outliers(uint32_t Nc, const CCWork &w, c
         __syncthreads();
     };
     const bool one_tile = n_out <= kStage;
-    if (one_tile) stage_tile(0u, n_out);
-    for (uint32_t k = wave; k < n_out; k += WG / 64) {
-        const uint32_t c = outs[k];
-        float2 p;
-        float ec;
-        if (one_tile) {
-            const float4 e = stage[k];
-            p = make_float2(e.x, e.y);
-            ec = e.z;
-        } else {
-            p = w.ref[c];
-            ec = crad[c] + w.skin[c];
-        }
+    // the cells a near outlier scans: (2R + 1)^2 around its own, R = ceil(reach)
+    auto near_side = [&](float ec) -> int {
         const double reach = reach_of(ec);
-        if (!(reach <= 8.0)) continue;
-        const int R = (int)ceil(reach), side = 2 * R + 1;
-        const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
-        for (int q = (int)lane; q < side * side; q += 64) {
-            const int cx = cx0 + (q % side) - R, cy = cy0 + (q / side) - R;
-            const uint32_t kk = cell_hash(cx, cy) & tmask;
+        return reach <= 8.0 ? 2 * (int)ceil(reach) + 1 : 0;
+    };
+    auto scan_cell = [&](uint32_t c, float2 p, float ec, int cx, int cy) {
+        const uint32_t kk = cell_hash(cx, cy) & tmask;
 #pragma unroll 4
-            for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
-                const float4 r = w.rec[s];
-                const uint32_t o = __float_as_uint(r.w);
-                if (o & kOutlierBit) continue;  // outliers: below
-                if (scell(r.x, inv_h) != cx || scell(r.y, inv_h) != cy) continue;  // another cell, same bucket
-                float ex = p.x - r.x, ey = p.y - r.y;
-                float lim = ec + r.z;
-                if (ex * ex + ey * ey < lim * lim) { add(c, o); add(o, c); }
+        for (uint32_t s = w.start[kk], s1 = w.start[kk + 1]; s < s1; s++) {
+            const float4 r = w.rec[s];
+            const uint32_t o = __float_as_uint(r.w);
+            if (o & kOutlierBit) continue;  // outliers: below
+            if (scell(r.x, inv_h) != cx || scell(r.y, inv_h) != cy) continue;  // another cell, same bucket
+            float ex = p.x - r.x, ey = p.y - r.y;
+            float lim = ec + r.z;
+            if (ex * ex + ey * ey < lim * lim) { add(c, o); add(o, c); }
+        }
+    };
+    if (one_tile && n_out) {
+        stage_tile(0u, n_out);
+        // every (near outlier, cell) item across the workgroup: each outlier's
+        // first item (an exclusive prefix over the outliers' cell counts) goes
+        // in its stage entry's w, and a thread takes a contiguous run of items,
+        // finding its first outlier by bisection
+        const uint32_t t = threadIdx.x;
+        uint32_t cells = 0;
+        if (t < n_out) {
+            const int side = near_side(stage[t].z);
+            cells = (uint32_t)(side * side);
+        }
+        const uint32_t inc = __ockl_wfscan_add_u32(cells, true);
+        if (lane == 63u) s_wave[wave] = inc;
+        __syncthreads();
+        const uint32_t uw = uni(wave);
+        uint32_t below = 0, items = 0;
+#pragma unroll 1
+        for (uint32_t v = 0; v < uw; v++) below += s_wave[v];
+        items = below;
+#pragma unroll 1
+        for (uint32_t v = uw; v < WG / 64; v++) items += s_wave[v];
+        if (t < n_out) stage[t].w = __uint_as_float(below + inc - cells);
+        __syncthreads();
+        const uint32_t per = (items + WG - 1) / WG;
+        uint32_t i = t * per;
+        const uint32_t i1 = min(items, i + per);
+        if (i < i1) {
+            // the last outlier whose first item is at or before i (one with no
+            // cells shares its first item with the next, and is passed over)
+            uint32_t lo = 0, hi = n_out;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (__float_as_uint(stage[mid].w) <= i) lo = mid; else hi = mid;
+            }
+            uint32_t k = lo;
+            float4 e = stage[k];
+            int side = near_side(e.z);
+            uint32_t q = i - __float_as_uint(e.w);
+#pragma unroll 1
+            for (; i < i1; i++) {
+                if (q == (uint32_t)(side * side)) {
+                    // the next outlier with cells
+                    do {
+                        e = stage[++k];
+                        side = near_side(e.z);
+                    } while (side == 0 && k + 1u < n_out);
+                    q = 0;
+                }
+                const float2 p = make_float2(e.x, e.y);
+                const int R = side >> 1;
+                const int cx = scell(p.x, inv_h) + (int)(q % (uint32_t)side) - R;
+                const int cy = scell(p.y, inv_h) + (int)(q / (uint32_t)side) - R;
+                scan_cell(outs[k], p, e.z, cx, cy);
+                q++;
             }
         }
+    }
+    // more outliers than one stage holds: a wavefront per outlier, its lanes
+    // over the cells
+    for (uint32_t k = wave; k < n_out && !one_tile; k += WG / 64) {
+        const uint32_t c = outs[k];
+        const float2 p = w.ref[c];
+        const float ec = crad[c] + w.skin[c];
+        const int side = near_side(ec);
+        if (side == 0) continue;
+        const int R = side >> 1;
+        const int cx0 = scell(p.x, inv_h), cy0 = scell(p.y, inv_h);
+        for (int q = (int)lane; q < side * side; q += 64)
+            scan_cell(c, p, ec, cx0 + (q % side) - R, cy0 + (q / side) - R);
     }
     uint32_t n_far = 0;
     for (uint32_t k = 0; k < n_out; k++) {
@@ -1268,7 +1327,7 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
             __syncthreads();
             const uint32_t n_far =
                 gather_outliers<WG, 0>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, nullptr,
-                                      (float4 *)s_wl);
+                                      (float4 *)s_wl, s_u32);
             __syncthreads();
             const uint64_t tb1 = wall_clock64();
             for (uint32_t c = tid; c < Nc; c += WG) w.off[c] = w.cnt[c];
@@ -1311,7 +1370,8 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
                 w.head[c] = w.off[c] + own;
             }
             __syncthreads();
-            gather_outliers<WG, 1>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, w.head, (float4 *)s_wl);
+            gather_outliers<WG, 1>(Nc, w, crad, sh->out, n_out, rmax, s_cell, h, inv_h, tmask, w.head, (float4 *)s_wl,
+                                   s_u32);
             __syncthreads();
             const uint64_t tb2 = wall_clock64();
             // ascending partner order (the reference's pair order): short lists by

@@ -171,6 +171,7 @@ def test_batch_dense_contact_exact():
         b.add_scene(s)
     oracles = [to_oracle(s, 32) for s in scenes]
     done = 0
+    n_out = 0
     for chunk in (60, 60, 60, 60):
         b.update(DT, chunk)
         b.sync()
@@ -178,6 +179,7 @@ def test_batch_dense_contact_exact():
             for _ in range(chunk):
                 assert o.update(DT) == 0
         done += chunk
+        n_out = max([n_out] + [b.scene_lists(i)["n_out"] for i in range(len(scenes))])
         for i, s in enumerate(scenes):
             b.read_scene(i, s)
             a, w = state(s), state(oracles[i])
@@ -186,7 +188,9 @@ def test_batch_dense_contact_exact():
     assert all(x["err"] == 0 for x in st)
     assert sum(x["rounds"] for x in st) > 0
     kept = sum(b.scene_lists(i)["kept"] for i in range(len(scenes)))
-    print("sweeps kept past skins:", kept, "redos:", sum(x["redos"] for x in st))
+    print("sweeps kept past skins:", kept, "redos:", sum(x["redos"] for x in st), "outliers seen:", n_out)
+    # the lists' outlier paths (circles whose skins outgrew the cells) ran
+    assert n_out > 0
 
 
 _placement_oracle = {}
