@@ -980,35 +980,22 @@ __device__ __forceinline__ void fallback_sweep(float2 *P, uint32_t Np, uint32_t 
     __syncthreads();
 }
 
-// The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
-// the sweep's list builds and the kernel's link and body phases keep their
-// registers.  Per circle, nxp (u16) is its next unprocessed partner, stp (u16)
-// the round in which it last moved and cur (u32) the entry of that partner
-// in the candidate array (its list offset plus its head).  With these in LDS
-// a round's list state is an LDS read, and the next candidate one global
-// read after it.  They sit in the LDS the scene leaves free (rs_lds), or, for
-// a sweep with enough candidate pairs to pay for two copies, in place of the
-// circles' previous positions (held in LDS, unused by the sweep) or of their
-// positions (the rounds then work on the global copy), parked in global
-// memory meanwhile; else in the workspace (key, stamp, head).  The heads go
-// back to the workspace at the end.
-// The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
-// the sweep's list builds and the kernel's link and body phases keep their
-// registers.  Per circle, nxp (u16) is its next unprocessed partner, stp (u16)
-// the round in which it last moved and cur (u32) the entry of that partner
-// in the candidate array (its list offset plus its head).  With these in LDS
-// a round's list state is an LDS read, and the next candidate one global
-// read after it.  They sit in the LDS the scene leaves free (rs_lds), or, for
-// a sweep with enough candidate pairs to pay for two copies, in place of the
-// circles' previous positions (held in LDS, unused by the sweep) or of their
-// positions (the rounds then work on the global copy), parked in global
-// memory meanwhile; else in the workspace (key, stamp, head).  The heads go
-// back to the workspace at the end.
-// The rounds themselves (see sweep_rounds); nxp, stp and cur point into LDS
-// or, typed as global, into the workspace.
+// The rounds themselves (see sweep_rounds).  Per circle: nxp (u16) its next
+// unprocessed partner, nx2 (u16) the partner after that, and cst (u32) the
+// entry of nxp in the candidate array (its list offset plus its head, low 24
+// bits) with the round in which the circle last moved (high 8 bits, kStamp8
+// for none).  nxp, nx2 and cst point into LDS or, typed as global, into the
+// workspace.  A thread's first pair of a round takes its new next partners
+// from nx2 and loads the entries after them from the candidate array after
+// the pair is solved; they are written to nx2 only at the end of the
+// readiness pass, so that load overlaps the barrier and the pass instead of
+// lengthening the round.
+constexpr uint32_t kCurMask = 0x00ffffffu;
+constexpr uint32_t kStamp8 = 0xffu;
+static_assert(kCandCap * 65536ull <= kCurMask + 1ull, "a list cursor must fit 24 bits");
 template <int WG, typename PCP, typename U16P, typename U32P>
 __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *crad_, bool uni_r, float runi,
-                                            CCShared *sh, uint32_t *s_wl, U16P nxp, U16P stp, U32P cur) {
+                                            CCShared *sh, uint32_t *s_wl, U16P nxp, U16P nx2, U32P cst) {
     const int tid = threadIdx.x;
     const CCWork &ww = sh->work;
     struct {
@@ -1030,17 +1017,16 @@ __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *cr
         if (i < (uint32_t)WG) s_wl[b * WG + i] = v; else gwl[b][i] = v;
     };
     auto wl_get = [&](uint32_t b, uint32_t i) { return i < (uint32_t)WG ? s_wl[b * WG + i] : gwl[b][i]; };
-    // the last candidate entry: a cursor's next entry is read before its
-    // list end is known, and only used below that end
+    // the last candidate entry: an entry past a list's end is read before
+    // that end is known, and only used below it
     const uint32_t cand_last = kCandCap * Nc - 1u;
 #pragma unroll 1
     for (uint32_t c = tid; c < Nc; c += WG) {
         w.save[c] = Pc[c];
-        w.path[c] = 0.0f;
-        const uint32_t o0 = w.off[c];
-        cur[c] = o0;
-        stp[c] = (uint16_t)kNone16;
-        nxp[c] = w.cnt[c] ? w.cand[o0] : (uint16_t)kNone16;
+        const uint32_t o0 = w.off[c], n0 = w.cnt[c];
+        cst[c] = o0 | (kStamp8 << 24);
+        nxp[c] = n0 > 0u ? w.cand[o0] : (uint16_t)kNone16;
+        nx2[c] = n0 > 1u ? w.cand[o0 + 1u] : (uint16_t)kNone16;
     }
     if (tid == 0) { sh->cnt3[0] = 0u; sh->cnt3[1] = 0u; sh->cnt3[2] = 0u; }
     __syncthreads();
@@ -1051,46 +1037,87 @@ __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *cr
     }
     __syncthreads();
     uint32_t n = uni(sh->cnt3[0]);
-    uint32_t r = 0, r16 = 0;
+    uint32_t r = 0, r8 = 0;
+    // one pair: solve it, move both heads to their next partners (from nx2),
+    // and between the solve and the stores call far(uc, uo) with the pair's
+    // cursors, to issue the loads for the partners after next: vector memory
+    // operations complete in order, stores included, so loads issued after
+    // the stores would have to wait for them
+    auto apply = [&](uint32_t c, uint32_t o, uint32_t &xc, uint32_t &xo, auto &&far) {
+        float2 pi = f32x2(Pc[c]), pj = f32x2(Pc[o]);
+        const uint32_t uc = cst[c] & kCurMask, uo = cst[o] & kCurMask;
+        xc = nx2[c];
+        xo = nx2[o];
+        float qi = 0.0f, qj = 0.0f;
+        const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
+        const bool touch = solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj);
+        __builtin_amdgcn_sched_barrier(0);
+        far(uc, uo);
+        __builtin_amdgcn_sched_barrier(0);
+        if (touch) {
+            Pc[c] = bits(pi);
+            Pc[o] = bits(pj);
+        }
+        // each circle's position after each of its pairs, in list order: its
+        // excursion from the sweep's start is taken from these at the end
+        w.traj[uc] = bits(pi);
+        w.traj[uo] = bits(pj);
+        cst[c] = (uc + 1u) | (r8 << 24);
+        cst[o] = (uo + 1u) | (r8 << 24);
+        nxp[c] = (uint16_t)xc;
+        nxp[o] = (uint16_t)xo;
+    };
 #pragma unroll 1
     for (; n > 0; r++) {
         const uint32_t cb = r & 1u, nb = cb ^ 1u;
-        // apply the ready pairs and move both heads; a thread keeps its first
-        // pair and the two new next partners for the readiness check
+        // a thread's first pair: its circles, their new next partners, and
+        // the loads (list ends and entries) for the partners after them
         uint32_t mc = 0, mo = 0, mnc = kNone16, mno = kNone16;
+        // set and read only under `first`: left without a value otherwise, so
+        // that no merge of the two paths has to wait for the loads
+        uint32_t fec, feo, fuc, fuo;
+        uint16_t fyc, fyo;  // as loaded: no widening to wait for
+        // further pairs (more ready pairs than threads) complete at once, and
+        // before the first pair, whose loads nothing after them may wait for
 #pragma unroll 1
-        for (uint32_t k = tid; k < n; k += WG) {
+        for (uint32_t k = tid + WG; k < n; k += WG) {
             const uint32_t pr = wl_get(cb, k), c = pr & 0xffffu, o = pr >> 16;
-            float2 pi = f32x2(Pc[c]), pj = f32x2(Pc[o]);
-            const uint32_t uc = cur[c], uo = cur[o];
-            const uint32_t ec = w.off[c + 1], eo = w.off[o + 1];  // list ends
-            const uint32_t yc = w.cand[min(uc + 1u, cand_last)], yo = w.cand[min(uo + 1u, cand_last)];
-            const uint32_t xc = uc + 1u < ec ? yc : kNone16;
-            const uint32_t xo = uo + 1u < eo ? yo : kNone16;
-            float qi = 0.0f, qj = 0.0f;
-            const float rc = uni_r ? runi : crad[c], ro = uni_r ? runi : crad[o];
-            // loaded beside the list state, not after the test: one global
-            // latency less per round for a pair that touches
-            const float pathc = w.path[c], patho = w.path[o];
-            const float2 savec = f32x2(w.save[c]), saveo = f32x2(w.save[o]);
-            if (solve_circle_pair(pi.x, pi.y, pj.x, pj.y, rc, ro, qi, qj)) {
-                Pc[c] = bits(pi);
-                Pc[o] = bits(pj);
-                w.path[c] = excursion(pathc, pi.x, pi.y, savec.x, savec.y);
-                w.path[o] = excursion(patho, pj.x, pj.y, saveo.x, saveo.y);
-            }
-            w.traj[uc] = bits(pi);
-            w.traj[uo] = bits(pj);
-            cur[c] = uc + 1u;
-            cur[o] = uo + 1u;
-            nxp[c] = (uint16_t)xc;
-            nxp[o] = (uint16_t)xo;
-            stp[c] = (uint16_t)r16;
-            stp[o] = (uint16_t)r16;
-            if (k == (uint32_t)tid) { mc = c; mo = o; mnc = xc; mno = xo; }
+            uint32_t xc, xo, uc = 0, uo = 0, ec = 0, eo = 0, yc = 0, yo = 0;
+            apply(c, o, xc, xo, [&](uint32_t ucc, uint32_t uoo) {
+                uc = ucc;
+                uo = uoo;
+                ec = w.off[c + 1];
+                eo = w.off[o + 1];
+                yc = w.cand[min(ucc + 2u, cand_last)];
+                yo = w.cand[min(uoo + 2u, cand_last)];
+            });
+            nx2[c] = uc + 2u < ec ? (uint16_t)yc : (uint16_t)kNone16;
+            nx2[o] = uo + 2u < eo ? (uint16_t)yo : (uint16_t)kNone16;
+        }
+        const bool first = (uint32_t)tid < n;
+        if (first) {
+            const uint32_t pr = wl_get(cb, tid);
+            mc = pr & 0xffffu;
+            mo = pr >> 16;
+            apply(mc, mo, mnc, mno, [&](uint32_t uc, uint32_t uo) {
+                fuc = uc;
+                fuo = uo;
+                fec = w.off[mc + 1];
+                feo = w.off[mo + 1];
+                fyc = w.cand[min(uc + 2u, cand_last)];
+                fyo = w.cand[min(uo + 2u, cand_last)];
+            });
         }
         if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;  // read at the end of round r-1, two barriers ago
         __syncthreads();
+        if (first) {
+            // the first pair's partners after next, read across the barrier
+            // (no one reads nx2 before the next round); written before the
+            // readiness pass, whose loop a wait could not count through, so
+            // the wait leaves the pair's stores in flight
+            nx2[mc] = fuc + 2u < fec ? fyc : (uint16_t)kNone16;
+            nx2[mo] = fuo + 2u < feo ? fyo : (uint16_t)kNone16;
+        }
         // pairs completed by a moved head
 #pragma unroll 1
         for (uint32_t k = tid; k < n; k += WG) {
@@ -1104,7 +1131,7 @@ __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *cr
             for (int e = 0; e < 2; e++) {
                 const uint32_t x = e ? o : c, y = e ? no : nc;
                 if (y == kNone16 || nxp[y] != x) continue;
-                if (stp[y] == r16 && y < x) continue;  // both moved: the smaller one adds it
+                if ((cst[y] >> 24) == r8 && y < x) continue;  // both moved: the smaller one adds it
                 const uint32_t a = x < y ? x : y, b = x < y ? y : x;
                 wl_put(nb, atomicAdd(&sh->cnt3[(r + 1) % 3], 1u), a | (b << 16));
             }
@@ -1112,25 +1139,39 @@ __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *cr
         __syncthreads();
         n = uni(sh->cnt3[(r + 1) % 3]);
         if (tid == 0) sh->cnt.rounds++;
-        if (++r16 == 65535u) {
+        if (++r8 == kStamp8) {
             // every round number is about to recur: no stamp may match one
 #pragma unroll 1
-            for (uint32_t c = tid; c < Nc; c += WG) stp[c] = (uint16_t)kNone16;
+            for (uint32_t c = tid; c < Nc; c += WG) cst[c] = cst[c] | (kStamp8 << 24);
             __syncthreads();
-            r16 = 0u;
+            r8 = 0u;
         }
     }
+    // heads, and each circle's largest excursion from its start position
+    // (the running maximum over its positions after its pairs, in their
+    // order: a pair that did not touch repeats the position before it)
 #pragma unroll 1
-    for (uint32_t c = tid; c < Nc; c += WG) w.head[c] = cur[c] - w.off[c];
+    for (uint32_t c = tid; c < Nc; c += WG) {
+        const uint32_t o0 = w.off[c], u1 = cst[c] & kCurMask;
+        w.head[c] = u1 - o0;
+        const float2 sv = f32x2(w.save[c]);
+        float pth = 0.0f;
+#pragma unroll 4
+        for (uint32_t u = o0; u < u1; u++) {
+            const float2 q = f32x2(w.traj[u]);
+            pth = excursion(pth, q.x, q.y, sv.x, sv.y);
+        }
+        w.path[c] = pth;
+    }
 }
 
 // The pair rounds of one circle-pair sweep (see circle_sweep), out of line:
 // the sweep's list builds and the kernel's link and body phases keep their
-// registers.  Per circle, nxp (u16) is its next unprocessed partner, stp (u16)
-// the round in which it last moved and cur (u32) the entry of that partner
-// in the candidate array (its list offset plus its head).  With these in LDS
-// a round's list state is an LDS read, and the next candidate one global
-// read after it.  They sit in the LDS the scene leaves free (rs_off bytes
+// registers.  The round state is 8 bytes a circle: its next two unprocessed
+// partners and its list cursor with the round it last moved in (see
+// rounds_core).  With it in LDS a round's list state is an LDS read, and the
+// candidate load it needs overlaps the round's barrier.  It sits in the LDS
+// the scene leaves free (rs_off bytes
 // into the dynamic LDS, kNone for none), or, for a sweep with enough
 // candidate pairs to pay for two copies, in place of the circles' previous
 // positions (held in LDS, unused by the sweep) or of their positions (the
