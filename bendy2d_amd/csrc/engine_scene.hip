@@ -50,6 +50,15 @@ struct ColliderStage {
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float uni(float v) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v))); }
 
+// Orders one wavefront's memory operations between its lanes: a wavefront's
+// LDS and vector memory instructions execute in order, so keeping the
+// compiler from moving them across this point is all that is needed.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // A uniform pointer into global memory, typed as such, with its base read
 // from the first lane: loads through it take a 32-bit offset beside a scalar
 // base instead of a 64-bit flat address.
@@ -1067,8 +1076,72 @@ __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *cr
         nxp[c] = (uint16_t)xc;
         nxp[o] = (uint16_t)xo;
     };
+    // rounds with few ready pairs: the first wavefront runs them alone, in
+    // step, with no workgroup barrier, while the count stays within it (the
+    // same steps as a workgroup round below, for lanes < n <= 64); the
+    // others wait, then take the round number and count from LDS
+    __shared__ uint32_t s_rr[2];
+    auto wave_rounds = [&]() {
 #pragma unroll 1
-    for (; n > 0; r++) {
+        do {
+            const uint32_t cb = r & 1u, nb = cb ^ 1u;
+            const bool first = (uint32_t)tid < n;
+            uint32_t mc = 0, mo = 0, mnc = kNone16, mno = kNone16;
+            uint32_t fec, feo, fuc, fuo;
+            uint16_t fyc, fyo;
+            if (first) {
+                const uint32_t pr = s_wl[cb * WG + tid];
+                mc = pr & 0xffffu;
+                mo = pr >> 16;
+                apply(mc, mo, mnc, mno, [&](uint32_t uc, uint32_t uo) {
+                    fuc = uc;
+                    fuo = uo;
+                    fec = w.off[mc + 1];
+                    feo = w.off[mo + 1];
+                    fyc = w.cand[min(uc + 2u, cand_last)];
+                    fyo = w.cand[min(uo + 2u, cand_last)];
+                });
+            }
+            if (tid == 0) sh->cnt3[(r + 2) % 3] = 0u;
+            wave_sync();
+            if (first) {
+#pragma unroll 1
+                for (int e = 0; e < 2; e++) {
+                    const uint32_t x = e ? mo : mc, y = e ? mno : mnc;
+                    if (y == kNone16 || nxp[y] != x) continue;
+                    if ((cst[y] >> 24) == r8 && y < x) continue;  // both moved: the smaller one adds it
+                    const uint32_t a = x < y ? x : y, b = x < y ? y : x;
+                    wl_put(nb, atomicAdd(&sh->cnt3[(r + 1) % 3], 1u), a | (b << 16));
+                }
+                nx2[mc] = fuc + 2u < fec ? fyc : (uint16_t)kNone16;
+                nx2[mo] = fuo + 2u < feo ? fyo : (uint16_t)kNone16;
+            }
+            wave_sync();
+            n = uni(sh->cnt3[(r + 1) % 3]);
+            if (tid == 0) sh->cnt.rounds++;
+            if (++r8 == kStamp8) {
+#pragma unroll 1
+                for (uint32_t c = tid; c < Nc; c += 64) cst[c] = cst[c] | (kStamp8 << 24);
+                wave_sync();
+                r8 = 0u;
+            }
+            r++;
+        } while (n > 0 && n <= 64u);
+    };
+#pragma unroll 1
+    while (n > 0) {
+        if (n <= 64u) {
+            if (tid < 64) {
+                wave_rounds();
+                if (tid == 0) { s_rr[0] = r; s_rr[1] = r8; }
+            }
+            __syncthreads();
+            r = uni(s_rr[0]);
+            r8 = uni(s_rr[1]);
+            n = uni(sh->cnt3[r % 3]);
+            __syncthreads();  // s_rr is read before a later pass of the first wavefront writes it
+            continue;
+        }
         const uint32_t cb = r & 1u, nb = cb ^ 1u;
         // a thread's first pair: its circles, their new next partners, and
         // the loads (list ends and entries) for the partners after them
@@ -1146,6 +1219,7 @@ __device__ __forceinline__ void rounds_core(PCP Pc, uint32_t Nc, const float *cr
             __syncthreads();
             r8 = 0u;
         }
+        r++;
     }
     // heads, and each circle's largest excursion from its start position
     // (the running maximum over its positions after its pairs, in their
