@@ -1785,15 +1785,20 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
             auto link_sweeps = [&](const LinkS *lk) {
             for (uint32_t it = 0; it < d.link_iters; it++) {
                 if (npoly && it + 1 == d.link_iters) poly_centres();
+                // with the records in global memory, each layer's first record
+                // for this thread loads before the barrier that ends the layer
+                // before it, so the load overlaps the barrier (a barrier waits
+                // for outstanding LDS reads, so records in LDS load after it)
+                LinkS nx0{0, 0, 0.0f};
+                if (!links_lds && d.depth && kloff[0] + tid < kloff[1]) nx0 = lk[kloff[0] + tid];
                 for (uint32_t L = 0; L < d.depth; L++) {
                     uint32_t lo = kloff[L], hi = kloff[L + 1];
-                    // not unrolled (measured faster than unrolling by 2, 4 or 8,
-                    // and than loading the next layer's first record early:
-                    // register pressure of the whole persistent kernel); the
+                    // not unrolled (measured faster than unrolling by 2, 4 or
+                    // 8: register pressure of the whole persistent kernel); the
                     // thread's next record loads while this one is solved
                     uint32_t l = lo + tid;
-                    LinkS nxt{0, 0, 0.0f};
-                    if (l < hi) nxt = lk[l];
+                    LinkS nxt = nx0;
+                    if (links_lds && l < hi) nxt = lk[l];
 #pragma unroll 1
                     for (; l < hi; l += WG) {
                         const LinkS rec = nxt;
@@ -1809,6 +1814,10 @@ __device__ __forceinline__ void run_scene(const SceneDesc *__restrict__ dp, uint
                         }
                         P[a] = A;
                         P[b] = B;
+                    }
+                    if (!links_lds && L + 1 < d.depth) {
+                        const uint32_t l2 = hi + tid;
+                        nx0 = l2 < kloff[L + 2] ? lk[l2] : LinkS{0, 0, 0.0f};
                     }
                     __syncthreads();
                 }
