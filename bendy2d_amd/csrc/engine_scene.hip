@@ -1553,9 +1553,10 @@ __device__ uint32_t circle_sweep(float2 *P, uint32_t Np, uint32_t Nc, const floa
         }
         // ---- sweep: worklist rounds.  nxp: each circle's next unprocessed
         // partner, kNone16 when its list is done, so a pair (x, y) is ready
-        // exactly when nxp[x] == y and nxp[y] == x; stp: the round (counted
-        // modulo 65535, with a clear at each wrap) in which the circle last
-        // moved (see sweep_rounds).
+        // exactly when nxp[x] == y and nxp[y] == x; with it each circle keeps
+        // the partner after that, its list cursor, and the round (counted
+        // modulo 255, with a clear at each wrap) in which it last moved (see
+        // rounds_core).
         sweep_rounds<WG>(P, Np, Nc, crad, uni_r, runi, sh, s_wl, rs_off, cs.pairs);
         // every list must be exhausted (a schedule that stalls is an error);
         // circles beyond 0.8 of their skin go to out[] (free after list builds)
