@@ -95,6 +95,20 @@ void BatchCore::build_order(const std::vector<uint32_t> &active) {
     invalidate_graph();
 }
 
+void BatchCore::order_by(const std::vector<uint64_t> &cost) {
+    auto by = [&](uint32_t a, uint32_t b) {
+        if (cost[a] != cost[b]) return cost[a] > cost[b];
+        uint64_t ca = (uint64_t)descs[a].N + descs[a].L, cb = (uint64_t)descs[b].N + descs[b].L;
+        return ca != cb ? ca > cb : a < b;
+    };
+    auto cls = order.begin();
+    for (uint32_t n : {n_tiny, n_small, n_large}) {
+        std::sort(cls, cls + n, by);
+        cls += n;
+    }
+    order_dirty = true;
+}
+
 int BatchCore::upload_tables(std::string &err) {
     size_t nd = std::max<size_t>(descs.size(), 1);
     if (d_desc_cap < nd) {
@@ -316,6 +330,41 @@ int64_t bd_batch_add_scene(bd_batch *b, const bd_solver *scene) {
 uint64_t bd_batch_len(const bd_batch *b) { return b ? b->images.size() : 0; }
 uint64_t bd_batch_total_points(const bd_batch *b) { return b ? b->total_points : 0; }
 
+// Each class's queue hands out the longest scenes first: by bodies plus
+// links until every scene has run, then by the device time each scene took
+// since the previous ranking (its phase timers), which also counts what that
+// estimate misses (circle sweeps, list rebuilds, dependency depth).  Sets
+// *contact when the scenes ran more than 64 pair rounds each on average
+// since then: their costs then change as they settle, and a long update call
+// ranks them again every kRerankLaunches launches.
+constexpr uint32_t kRerankLaunches = 8;
+static int order_by_measured_time(bd_batch *b, std::string &err, bool *contact) {
+    BatchCore &c = b->core;
+    const size_t n = b->images.size();
+    *contact = false;
+    if (!n || c.order.size() != n) return BD_OK;
+    std::vector<SceneState> st(n);
+    BD_HIP(hipMemcpyAsync(st.data(), b->states.p, n * sizeof(SceneState), hipMemcpyDeviceToHost, c.stream));
+    BD_HIP(hipStreamSynchronize(c.stream));
+    if (c.order_ticks.size() != n) c.order_ticks.assign(n, 0);
+    if (c.order_rounds.size() != n) c.order_rounds.assign(n, 0u);
+    std::vector<uint64_t> cost(n);
+    uint64_t rounds = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t t = st[i].t_links + st[i].t_build + st[i].t_sweep + st[i].t_bodies;
+        if (t <= c.order_ticks[i]) return BD_OK;
+        cost[i] = t - c.order_ticks[i];
+        rounds += st[i].rounds - c.order_rounds[i];
+    }
+    for (size_t i = 0; i < n; i++) {
+        c.order_ticks[i] += cost[i];
+        c.order_rounds[i] = st[i].rounds;
+    }
+    *contact = rounds > 64ull * n;
+    c.order_by(cost);
+    return BD_OK;
+}
+
 int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps) {
     if (!b) return BD_ERR_ARG;
     std::string err;
@@ -323,6 +372,9 @@ int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps) {
     if (rc) return fail(rc, err);
     if (!steps || b->images.empty()) return BD_OK;
     BatchCore &c = b->core;
+    bool contact = false;
+    rc = order_by_measured_time(b, err, &contact);
+    if (rc) return fail(rc, err);
     // descriptors depend on dt and on the pending initial acc
     bool any_acc = false;
     for (uint8_t a : b->acc_pending) any_acc |= a != 0;
@@ -349,7 +401,17 @@ int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps) {
         c.descs_dirty = true;
         steps -= 1;
     }
-    if (full) { rc = c.run(full, spl, err, !any_acc, !rem); if (rc) return fail(rc, err); }
+    for (uint32_t done = 0; done < full;) {
+        // in contact, a long call ranks the scenes again between chunks
+        const uint32_t chunk = contact ? std::min(full - done, kRerankLaunches) : full - done;
+        rc = c.run(chunk, spl, err, !any_acc && done == 0, !rem && done + chunk == full);
+        if (rc) return fail(rc, err);
+        done += chunk;
+        if (contact && done < full) {
+            rc = order_by_measured_time(b, err, &contact);
+            if (rc) return fail(rc, err);
+        }
+    }
     if (rem) { rc = c.run(1, rem, err, !any_acc && !full, true); if (rc) return fail(rc, err); }
     for (auto &u : b->updates_run) u += steps + (any_acc ? 1 : 0);
     return BD_OK;

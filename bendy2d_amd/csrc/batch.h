@@ -20,6 +20,8 @@ struct BatchCore {
     bd_batch_config cfg{};
     std::vector<SceneDesc> descs;     // indexed by scene slot
     std::vector<uint32_t> order;      // tiny, small, then large class, each largest first
+    std::vector<uint64_t> order_ticks;   // per scene slot: its device ticks at the last ordering
+    std::vector<uint32_t> order_rounds;  // per scene slot: its pair rounds at the last ordering
     uint32_t n_tiny = 0, n_small = 0, n_large = 0;
     bool descs_dirty = true, order_dirty = true;
     DevBuf<uint8_t> d_desc, d_order, d_queue;
@@ -40,6 +42,9 @@ struct BatchCore {
     int init_device(std::string &err);
     void invalidate_graph();
     void build_order(const std::vector<uint32_t> &active);
+    // the same classes, each ordered by cost[slot] (largest first, ties by
+    // bodies plus links); the launch graph stays valid
+    void order_by(const std::vector<uint64_t> &cost);
     int upload_tables(std::string &err);
     int enqueue_launch(uint32_t steps, std::string &err);
     // `launches` launches of `steps` updates each.  The device time between
