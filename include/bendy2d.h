@@ -290,8 +290,9 @@ int64_t bd_plan_layers(const uint32_t *a, const uint32_t *b, uint64_t n_links, u
  * holds copies of independent scenes (particles, circles, soft polygons and
  * static colliders; at most 16384 bodies each) resident in device memory; one persistent kernel launch per size
  * class advances all of them, one workgroup per scene taken from a work
- * queue, largest first.  Results per scene are bit-identical to bd_update on
- * the same scene. */
+ * queue, longest first (by bodies plus links, then by each scene's measured
+ * device time).  Results per scene are bit-identical to bd_update on the
+ * same scene. */
 typedef struct bd_batch bd_batch;
 
 /* Launch configuration.  Scenes with at most split_points bodies run on
@@ -318,7 +319,8 @@ int64_t bd_batch_add_scene(bd_batch *b, const bd_solver *scene);
 uint64_t bd_batch_len(const bd_batch *b);
 uint64_t bd_batch_total_points(const bd_batch *b);
 /* Advance every scene by `steps` updates of dt (each update runs the scene's
- * own sub_steps and link iterations). */
+ * own sub_steps and link iterations).  Each call first reads the scenes'
+ * device timers back (one stream synchronisation) to rank the queues. */
 int32_t bd_batch_update(bd_batch *b, float dt, uint32_t steps);
 int32_t bd_batch_sync(bd_batch *b);
 /* Copy scene id's positions back into `out`, a solver holding the same scene
