@@ -337,7 +337,7 @@ uint64_t bd_batch_total_points(const bd_batch *b) { return b ? b->total_points :
 // *contact when the scenes ran more than 64 pair rounds each on average
 // since then: their costs then change as they settle, and a long update call
 // ranks them again every kRerankLaunches launches.
-constexpr uint32_t kRerankLaunches = 8;
+constexpr uint32_t kRerankLaunches = 4;
 static int order_by_measured_time(bd_batch *b, std::string &err, bool *contact) {
     BatchCore &c = b->core;
     const size_t n = b->images.size();
